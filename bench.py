@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Single-query SSSP benchmark of the MI355X MLMQ solver.
+
+A step is one exact single-source query (reset + persistent solve) on the
+resident graph.  Default workload: the 6,262,104-vertex road-like graph
+(seeded 3-NN points in the unit square, symmetrised, Morton-ordered, weights
+= scaled Euclidean length with the longest edge at 100,000), source = the
+vertex nearest the centroid, delta 200000, queue L1SLF_L2DQ.
+
+Prints ONE JSON line (rank 0).  value = GTEPS over the timed steps
+(E_r * steps / wall seconds, E_r = out-degree sum of reachable vertices);
+ms_per_step = wall ms per step.  "roofline" prices the persistent kernel's
+algorithmic bytes B = 16 V_r + 12 E_r per query against its HIP-event
+duration and the 8 TB/s HBM peak.  "cpu_baseline" is the CPU oracle
+(single-thread binary-heap Dijkstra) on the same graph and source.
+"""
+import argparse
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "single-query ms & GTEPS; batched queries/s + p50/p99 latency (exact dists)"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="road6m")
+    ap.add_argument("--delta", type=int, default=None)
+    ap.add_argument("--queue", default="L1SLF_L2DQ")
+    ap.add_argument("--work-blocks", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--l2-batch", type=int, default=8)
+    ap.add_argument("--min-read", type=int, default=8)
+    ap.add_argument("--buckets", type=int, default=16)
+    ap.add_argument("--window", type=int, default=4)
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="per-launch HBM bytes measured by rocprofv3 --pmc (optional)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "independent queries would shard by replicating the graph per GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from mlmq_tpds_withl3_amd import Solver, make_workload, default_params
+
+    t0 = time.time()
+    g, src, spec = make_workload(args.workload)
+    row, col, w = g.arrays()
+    log(f"workload {args.workload}: n={g.n} m={g.m} src={src} gen {time.time() - t0:.1f}s")
+    delta = args.delta if args.delta is not None else spec["delta"]
+    params = default_params(queue=args.queue, delta=delta, work_blocks=args.work_blocks,
+                            blocks_per_cu=args.blocks_per_cu, l2_batch=args.l2_batch,
+                            min_read=args.min_read, buckets=args.buckets,
+                            bucket_window=args.window, timeout_s=30.0)
+
+    solver = Solver(0)
+    solver.load_csr(row, col, w)
+    shape = solver.device_shape()
+    for i in range(args.warmup):
+        t = solver.solve(src, params)
+        log(f"warmup {i}: solve_ms={t['solve_ms']:.3f} device_ms={t['device_ms']:.3f}")
+
+    timings = []
+    wall0 = time.perf_counter()
+    for _ in range(args.steps):
+        timings.append(solver.solve(src, params))
+    wall = time.perf_counter() - wall0
+    dist = solver.dist()
+    log(f"timed {args.steps} steps in {wall * 1e3:.1f} ms")
+
+    # ---- CPU baseline leg: the oracle, also the correctness check ----
+    from oracle import oracle as O
+    c0 = time.perf_counter()
+    ref, e_r, v_r = O.dijkstra(row, col, w, src)
+    first_cpu = time.perf_counter() - c0
+    correct = bool(np.array_equal(dist, ref))
+    cpu_times = [first_cpu]
+    if not args.no_cpu_baseline:
+        for _ in range(max(0, args.cpu_runs - 1)):
+            c0 = time.perf_counter()
+            O.dijkstra(row, col, w, src)
+            cpu_times.append(time.perf_counter() - c0)
+    cpu_s = statistics.median(cpu_times)
+    log(f"cpu oracle {cpu_s * 1e3:.1f} ms (runs {len(cpu_times)}), correct={correct}")
+
+    solve_ms = [t["solve_ms"] for t in timings]
+    dev_ms = [t["device_ms"] for t in timings]
+    ms_per_step = wall * 1e3 / args.steps
+    gteps = e_r * args.steps / wall / 1e9
+    b_alg = 16 * v_r + 12 * e_r
+    dev_avg_s = statistics.mean(dev_ms) / 1e3
+    achieved = b_alg / dev_avg_s / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            tj = json.load(open(args.traffic_json))
+            key = f"{args.workload}:{args.queue}"
+            if key in tj:
+                traffic = tj[key]["bytes_per_launch"]
+        except (OSError, ValueError, KeyError):
+            traffic = None
+    last = timings[-1]
+    out = {
+        "metric": METRIC,
+        "value": round(gteps, 6),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "uint32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "vertices": g.n, "edges": g.m, "source": src,
+            "reachable_vertices": v_r, "reachable_edges": e_r,
+            "weight_scale": spec.get("weight_scale_applied"),
+            "seed": spec.get("seed"),
+            "queue": args.queue, "delta": delta,
+            "buckets": args.buckets, "bucket_window": args.window,
+            "l2_batch": args.l2_batch, "min_read": args.min_read,
+            "work_blocks": last["work_blocks"], "waves": last["waves"],
+            "cus": shape["cus"], "blocks_per_cu_max": shape["blocks_per_cu_max"],
+        },
+        "solve_ms": {"median": round(statistics.median(solve_ms), 4),
+                     "min": round(min(solve_ms), 4), "max": round(max(solve_ms), 4)},
+        "device_ms": {"median": round(statistics.median(dev_ms), 4),
+                      "mean": round(statistics.mean(dev_ms), 4)},
+        "work": {"relaxations": last["relaxations"], "atomics": last["atomics"],
+                 "improvements": last["improvements"], "l2_pushes": last["l2_pushes"],
+                 "l2_pops": last["l2_pops"], "stale": last["stale"],
+                 "batches": last["batches"],
+                 "inflation": round(last["relaxations"] / max(1, e_r), 4)},
+        "correct": correct,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 4),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 8),
+            "traffic": traffic,
+            "kernel": "sssp_persistent_kernel",
+            "bytes_per_launch": b_alg,
+        },
+        "cpu_baseline": {
+            "value": round(e_r / cpu_s / 1e9, 6),
+            "unit": "GTEPS",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"full {args.workload} query from source {src} by the CPU oracle "
+                       f"(binary-heap Dijkstra, 1 thread), median of {len(cpu_times)} runs: "
+                       f"{cpu_s * 1e3:.1f} ms"),
+            "ms": round(cpu_s * 1e3, 3),
+            "cpu": cpu_model(),
+            "nproc": os.cpu_count(),
+        },
+    }
+    print(json.dumps(out), flush=True)
+    solver.close()
+    if not correct:
+        sys.exit(3)
+
+
+if __name__ == "__main__":
+    main()

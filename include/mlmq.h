@@ -1,0 +1,155 @@
+/*
+ * mlmq.h -- C ABI of the MI355X (gfx950) MLMQ single-source shortest-path
+ * solver: a per-wavefront "small label first" (SLF) deque in LDS over a global
+ * delta-bucket queue, drained by persistent wavefronts that relax a CSR graph
+ * with atomicMin on 32-bit distances.
+ *
+ * Conventions
+ *   - Plain C types only.  Every call returns 0 on success or a negative
+ *     MLMQ_E* code; nothing calls exit().  mlmq_last_error() gives the text.
+ *   - All state lives in an mlmq_ctx: contexts are independent (one per host
+ *     thread; a context is not safe for concurrent use from two threads).
+ *   - Every call is synchronous with respect to the host.
+ *   - Distances are int32 with 0x7FFFFFFF (INT_MAX) for unreachable vertices,
+ *     equal per vertex to a CPU Dijkstra on the same graph.
+ *
+ * Reference interface replaced (paths relative to the reference root;
+ * SS = evidence/l3_supplement_20260921_v2/single_source):
+ *   mlmq_create / mlmq_destroy      <- the solver's file-scope globals and
+ *                                      their allocation, SS/SSSP/sssp_run.cu:21-30, 676-707
+ *   mlmq_load_gr                    <- CSRGraph::read + copy_to_gpu,
+ *                                      SSSP/csr_graph.cu:185-300, 427-438
+ *   mlmq_load_csr                   <- copy_to_gpu of a host CSR, SSSP/csr_graph.cu:427-438
+ *   mlmq_attach_device_csr          <- sssp_init(m, nnz, RowPtr_d, ColIdx_d, w_d),
+ *                                      SS/SSSP/sssp_run.cu:676-707 (caller-owned device CSR)
+ *   mlmq_solve                      <- sssp_run_adaptive(src, info) -> kernel_adaptive,
+ *                                      SS/SSSP/sssp_run.cu:888-976, 758-886
+ *                                      (reset = sssp_re_init :709-747 + reinit_host)
+ *   mlmq_copy_dist                  <- sssp_cp_data, SS/SSSP/sssp_run.cu:749-756
+ *                                      (caller-owned buffer, as SSSP/sssp_run.cu:10868-10877)
+ *   mlmq_params fields              <- mlmq_setup (core/include/adaptive.h:6-59) and the
+ *                                      env knobs BENCH_DELTA, MLMQ_WORK_BLOCKS,
+ *                                      BENCH_QUEUE (SS/SSSP/sssp_run.cu:896;
+ *                                      SSSP/sssp_run.cu:11109; SSSP/main.cu:437-442)
+ *   sssp_init / sssp_run_adaptive / sssp_cp_data
+ *                                   <- the exact reference driver ABI,
+ *                                      SS/SSSP/main.cu:15-17 (single implicit context)
+ */
+#ifndef MLMQ_H
+#define MLMQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLMQ_OK                 0
+#define MLMQ_E_INVALID        (-1)   /* bad argument or malformed graph */
+#define MLMQ_E_NOGRAPH        (-2)   /* solve/copy before a graph was loaded */
+#define MLMQ_E_HIP            (-3)   /* HIP runtime error */
+#define MLMQ_E_NOMEM          (-4)   /* device or host allocation failed */
+#define MLMQ_E_IO             (-5)   /* file open/read/write failed */
+#define MLMQ_E_QUEUE_OVERFLOW (-6)   /* an L2 bucket ring ran full; retry with more capacity */
+#define MLMQ_E_TIMEOUT        (-7)   /* the on-device watchdog stopped the solve */
+#define MLMQ_E_INTERNAL       (-8)   /* queue accounting did not balance after a solve */
+#define MLMQ_E_UNSUPPORTED    (-9)   /* queue mode or parameter not available */
+
+/* Queue modes (values follow the reference's enum mlmq_type,
+ * core/include/common.h:85-108). */
+#define MLMQ_QUEUE_L1V_L2DQ     0    /* not implemented */
+#define MLMQ_QUEUE_L1N_L2DQ     1    /* no L1: every push and pop goes through L2 */
+#define MLMQ_QUEUE_L1SLF_L2DQ  19    /* the reference's default path */
+
+typedef struct mlmq_ctx mlmq_ctx;
+
+typedef struct mlmq_params {
+    int32_t  queue_mode;      /* MLMQ_QUEUE_L1SLF_L2DQ (default) or MLMQ_QUEUE_L1N_L2DQ */
+    uint32_t delta;           /* bucket width; reference BENCH_DELTA, default 200000 */
+    int32_t  work_blocks;     /* persistent workgroups; 0 = one per CU x blocks_per_cu */
+    int32_t  blocks_per_cu;   /* used when work_blocks == 0; 0 = default (2) */
+    int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
+    int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */
+    int32_t  l2_batch;        /* records per L2 claim (l2_batch_size), 1..64, default 8 */
+    int32_t  min_read;        /* top up from L2 below this many (MIN_READ_GRA), default 8 */
+    int64_t  bucket_capacity; /* ring slots per bucket (rounded up to 2^k); 0 = auto */
+    double   timeout_s;       /* device watchdog; <= 0 means 60 s */
+    int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
+    int32_t  reserved[7];
+} mlmq_params;
+
+/* Per-solve statistics (all counts summed over wavefronts). */
+typedef struct mlmq_timing {
+    double   solve_ms;        /* host steady clock around the persistent launch */
+    double   device_ms;       /* HIP events around the persistent launch */
+    double   reset_ms;        /* per-query reset (device fill + seed) */
+    double   query_wall_ms;   /* reset + solve + nothing else (copy-out is separate) */
+    int32_t  work_blocks;     /* workgroups launched */
+    int32_t  waves;           /* persistent wavefronts launched */
+    int64_t  bucket_capacity; /* ring slots per bucket used */
+    uint64_t l2_pushes;       /* records reserved in L2 (incl. the seed) */
+    uint64_t l2_pops;         /* records taken from L2 */
+    uint64_t relaxations;     /* edges examined */
+    uint64_t atomics;         /* atomicMin issued */
+    uint64_t improvements;    /* atomicMin that lowered a distance (= records pushed) */
+    uint64_t stale;           /* popped records skipped as stale */
+    uint64_t batches;         /* processed record batches */
+    uint64_t idle_polls;      /* idle iterations of all waves */
+    uint64_t retries;         /* overflow retries performed by mlmq_solve */
+    uint64_t reserved[6];
+} mlmq_timing;
+
+/* Reference graph statistics struct (core/include/graph_info.h:8-20). */
+typedef struct mlmq_graph_info {
+    int32_t m;          /* vertices (the reference names them m) */
+    int32_t nnz;        /* edges */
+    float avg_nnz, max_nnz, dev_nnz;
+    float avg_weight, dev_weight, max_weight;
+    float diameter;     /* -1: no estimate */
+} mlmq_graph_info;
+
+void        mlmq_params_default(mlmq_params *p);
+int         mlmq_create(int device, mlmq_ctx **out);
+int         mlmq_destroy(mlmq_ctx *ctx);
+const char *mlmq_last_error(const mlmq_ctx *ctx);
+const char *mlmq_version(void);
+
+/* Graph upload.  Host arrays are copied; `row` has n+1 entries with row[0] = 0
+ * and row[n] = m, `col` holds destinations in [0, n), `w` weights >= 0. */
+int mlmq_load_csr(mlmq_ctx *ctx, int32_t n, int64_t m, const int64_t *row,
+                  const int32_t *col, const int32_t *w);
+int mlmq_load_gr(mlmq_ctx *ctx, const char *path);
+/* Use a caller-owned device CSR (int32 row[n+1], col[m], w[m]) without a copy.
+ * The arrays must stay valid until another graph is loaded or the context is
+ * destroyed. */
+int mlmq_attach_device_csr(mlmq_ctx *ctx, int32_t n, int64_t m,
+                           const int32_t *row_d, const int32_t *col_d, const int32_t *w_d);
+int mlmq_graph_stats(mlmq_ctx *ctx, mlmq_graph_info *info);
+int mlmq_graph_size(const mlmq_ctx *ctx, int32_t *n, int64_t *m);
+
+/* One query: reset, seed `src`, run the persistent solver to termination.
+ * `params` may be NULL (defaults); `timing` may be NULL. */
+int mlmq_solve(mlmq_ctx *ctx, int32_t src, const mlmq_params *params, mlmq_timing *timing);
+/* Copy the n distances of the last solve into host memory (caller-owned). */
+int mlmq_copy_dist(mlmq_ctx *ctx, int32_t *host_out);
+/* Device pointer to the distance array of the last solve (n int32). */
+int mlmq_dist_device_ptr(mlmq_ctx *ctx, const int32_t **out);
+
+/* Occupancy query: resident workgroups per CU for the solver kernel and CU count. */
+int mlmq_device_shape(mlmq_ctx *ctx, int32_t *blocks_per_cu_max, int32_t *cus,
+                      int32_t *threads_per_block);
+
+/* ---- Reference driver ABI (SS/SSSP/main.cu:15-17), one implicit context ----
+ * sssp_init takes a device CSR in the reference layout (int32, row[0] = 0).
+ * sssp_run_adaptive reads BENCH_DELTA / MLMQ_WORK_BLOCKS from the environment
+ * like the reference and returns 0 or a negative MLMQ_E* code.
+ * sssp_cp_data malloc()s an m-entry array the caller frees. */
+int sssp_init(int m, int nnz, int *RowPtr_d, int *ColIdx_d, int *w_d);
+int sssp_run_adaptive(int src, mlmq_graph_info info);
+int sssp_cp_data(int **node_data_h);
+int sssp_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLMQ_H */

@@ -1,0 +1,148 @@
+"""ctypes binding of ``libmlmq.so`` (include/mlmq.h, include/mlmq_graph.h).
+
+The library is built in-tree by ``make -C mlmq_tpds_withl3_amd/csrc``; this
+module never builds or falls back to anything: if the shared object is missing
+the import of the solver fails with a RuntimeError naming the path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libmlmq.so")
+CLI_PATH = os.path.join(PKG_DIR, "mlmq")
+
+MLMQ_OK = 0
+ERRORS = {
+    -1: "MLMQ_E_INVALID", -2: "MLMQ_E_NOGRAPH", -3: "MLMQ_E_HIP", -4: "MLMQ_E_NOMEM",
+    -5: "MLMQ_E_IO", -6: "MLMQ_E_QUEUE_OVERFLOW", -7: "MLMQ_E_TIMEOUT",
+    -8: "MLMQ_E_INTERNAL", -9: "MLMQ_E_UNSUPPORTED",
+}
+QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1}
+
+# Every exported symbol the headers declare (checked by the CPU test suite).
+SOLVER_SYMBOLS = [
+    "mlmq_params_default", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
+    "mlmq_load_csr", "mlmq_load_gr", "mlmq_attach_device_csr", "mlmq_graph_stats",
+    "mlmq_graph_size", "mlmq_solve", "mlmq_copy_dist", "mlmq_dist_device_ptr",
+    "mlmq_device_shape", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
+]
+GRAPH_SYMBOLS = [
+    "mlmq_graph_generate_knn", "mlmq_graph_generate_rmat", "mlmq_graph_read_gr",
+    "mlmq_graph_from_csr", "mlmq_graph_write_gr", "mlmq_graph_dims", "mlmq_graph_arrays",
+    "mlmq_graph_coords", "mlmq_graph_weight_scale", "mlmq_graph_centroid_vertex", "mlmq_graph_max_degree_vertex",
+    "mlmq_graph_free",
+]
+
+
+class MLMQError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("queue_mode", ctypes.c_int32), ("delta", ctypes.c_uint32),
+        ("work_blocks", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+        ("buckets", ctypes.c_int32), ("bucket_window", ctypes.c_int32),
+        ("l2_batch", ctypes.c_int32), ("min_read", ctypes.c_int32),
+        ("bucket_capacity", ctypes.c_int64), ("timeout_s", ctypes.c_double),
+        ("auto_retry", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("solve_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+        ("reset_ms", ctypes.c_double), ("query_wall_ms", ctypes.c_double),
+        ("work_blocks", ctypes.c_int32), ("waves", ctypes.c_int32),
+        ("bucket_capacity", ctypes.c_int64),
+        ("l2_pushes", ctypes.c_uint64), ("l2_pops", ctypes.c_uint64),
+        ("relaxations", ctypes.c_uint64), ("atomics", ctypes.c_uint64),
+        ("improvements", ctypes.c_uint64), ("stale", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64), ("idle_polls", ctypes.c_uint64),
+        ("retries", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 6),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class GraphInfo(ctypes.Structure):
+    """Mirror of the reference's graph_info (core/include/graph_info.h:8-20)."""
+    _fields_ = [
+        ("m", ctypes.c_int32), ("nnz", ctypes.c_int32),
+        ("avg_nnz", ctypes.c_float), ("max_nnz", ctypes.c_float), ("dev_nnz", ctypes.c_float),
+        ("avg_weight", ctypes.c_float), ("dev_weight", ctypes.c_float),
+        ("max_weight", ctypes.c_float), ("diameter", ctypes.c_float),
+    ]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libmlmq.so; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libmlmq.so not built: {LIB_PATH} "
+                           "(run __graft_entry__.build() or make -C mlmq_tpds_withl3_amd/csrc)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    i32, i64, u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+    sig = {
+        "mlmq_params_default": (None, [ctypes.POINTER(Params)]),
+        "mlmq_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "mlmq_destroy": (ctypes.c_int, [P]),
+        "mlmq_last_error": (ctypes.c_char_p, [P]),
+        "mlmq_version": (ctypes.c_char_p, []),
+        "mlmq_load_csr": (ctypes.c_int, [P, i32, i64, P, P, P]),
+        "mlmq_load_gr": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "mlmq_attach_device_csr": (ctypes.c_int, [P, i32, i64, P, P, P]),
+        "mlmq_graph_stats": (ctypes.c_int, [P, ctypes.POINTER(GraphInfo)]),
+        "mlmq_graph_size": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i64)]),
+        "mlmq_solve": (ctypes.c_int, [P, i32, ctypes.POINTER(Params), ctypes.POINTER(Timing)]),
+        "mlmq_copy_dist": (ctypes.c_int, [P, P]),
+        "mlmq_dist_device_ptr": (ctypes.c_int, [P, ctypes.POINTER(P)]),
+        "mlmq_device_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                             ctypes.POINTER(i32)]),
+        "sssp_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P, P, P]),
+        "sssp_run_adaptive": (ctypes.c_int, [ctypes.c_int, GraphInfo]),
+        "sssp_cp_data": (ctypes.c_int, [ctypes.POINTER(ctypes.POINTER(ctypes.c_int))]),
+        "sssp_release": (ctypes.c_int, []),
+        "mlmq_graph_generate_knn": (ctypes.c_int, [i32, i32, u64, ctypes.c_double, i32,
+                                                   ctypes.POINTER(P)]),
+        "mlmq_graph_generate_rmat": (ctypes.c_int, [i32, i32, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, u64, i32, i32,
+                                                    ctypes.POINTER(P)]),
+        "mlmq_graph_read_gr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(P)]),
+        "mlmq_graph_from_csr": (ctypes.c_int, [i32, i64, P, P, P, ctypes.POINTER(P)]),
+        "mlmq_graph_write_gr": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "mlmq_graph_dims": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i64)]),
+        "mlmq_graph_arrays": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P),
+                                             ctypes.POINTER(P)]),
+        "mlmq_graph_coords": (ctypes.c_int, [P, ctypes.POINTER(P)]),
+        "mlmq_graph_weight_scale": (ctypes.c_double, [P]),
+        "mlmq_graph_centroid_vertex": (i32, [P]),
+        "mlmq_graph_max_degree_vertex": (i32, [P]),
+        "mlmq_graph_free": (None, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != MLMQ_OK:
+        msg = ""
+        if ctx is not None:
+            raw = load().mlmq_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise MLMQError(rc, msg)

@@ -1,0 +1,183 @@
+// L2: the global delta-bucket queue, wave64 edition.
+//
+// Behaviour follows the reference's l2_delta_queue
+// (core/cu_delta_queue/cu_delta_queue.cuh:78-642): records sit in `nbuckets`
+// circular buckets of width `delta`; the readable window is the `window`
+// buckets starting at the epoch `base` (the reference's first_pos); a write
+// goes to bucket clamp(dist / delta - base, 0, nbuckets - 1) relative to the
+// window (:443-451); readers hold a persistent per-bucket claim of up to
+// `batch` slots (:323-327), stop after the first bucket that yields records
+// (DQ_READ_MIN 1, :298), and the window advances when its head bucket is
+// empty while the queue is not (:374-382).
+//
+// What is different, by design for gfx950:
+//  * Publication is per slot.  A slot holds one 8-byte record or kEmpty; the
+//    writer installs its record with a 64-bit compare-and-swap from kEmpty and
+//    the reader takes it with an exchange back to kEmpty.  There is no
+//    block_write_done array, no manager warps publishing read_pos and no
+//    per-block read_pos mirror: a reader polls its own claimed slots.  The
+//    record itself is the publication flag, so no fence chain between payload
+//    and counter is needed (the chain the reference could not prove).
+//  * A full slot (the ring lapped an unconsumed record) makes the writer wait
+//    with a bounded spin; past the bound it raises kErrOverflow and the solve
+//    stops, instead of silently overwriting.
+//  * A claim is (start, 64-bit pending mask) held in lane b of the wave for
+//    bucket b, so a wave keeps claims on all buckets in two registers.
+//  * Counters live on separate 128-byte lines; all are agent-scope atomics.
+#pragma once
+
+#include "device_common.h"
+
+namespace mlmq {
+
+struct L2Queue {
+    uint64_t *slots;     // nbuckets << cap_log2 records
+    uint32_t *ctl;       // control block (device_common.h layout)
+    uint32_t nbuckets;   // 1..64
+    uint32_t window;     // 1..nbuckets
+    uint32_t cap_log2;   // ring slots per bucket = 1 << cap_log2
+    uint32_t delta;      // bucket width (> 0)
+    uint32_t batch;      // claim size, 1..64
+    uint32_t spin_limit; // bounded wait on a full slot
+};
+
+struct WaveStats {
+    uint64_t v[kNumStats];
+};
+
+__device__ __forceinline__ uint64_t *slot_ptr(const L2Queue &q, uint32_t b, uint32_t pos) {
+    return q.slots + ((uint64_t)b << q.cap_log2) + (pos & ((1u << q.cap_log2) - 1));
+}
+
+__device__ __forceinline__ void raise_error(const L2Queue &q, uint32_t code) {
+    if (lane_id() == 0) {
+        cas32(q.ctl + kErrOff, kErrNone, code);
+        st_relaxed(q.ctl + kExitOff, 1u);
+    }
+}
+
+// Push the records of the lanes with `valid` set.  Wave-uniform call.
+// Returns false when a slot stayed full past the spin bound.
+__device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &st) {
+    uint64_t todo = ballot(valid);
+    if (todo == 0) return true;
+    const uint32_t lane = lane_id();
+    const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
+    uint32_t b = 0;
+    if (valid) {
+        int32_t rel = (int32_t)(rec_dist(rec) / q.delta) - (int32_t)base;
+        rel = rel < 0 ? 0 : rel;
+        rel = rel >= (int32_t)q.nbuckets ? (int32_t)q.nbuckets - 1 : rel;
+        b = (base + (uint32_t)rel) % q.nbuckets;
+    }
+    st.v[kStatL2Push] += popc(todo);
+    bool ok = true;
+    while (todo) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t bl = read_lane(b, leader);
+        const bool in_grp = ((todo >> lane) & 1) && b == bl;
+        const uint64_t grp = ballot(in_grp);
+        uint32_t pos0 = 0;
+        if (lane == leader) pos0 = fetch_add(q.ctl + kTailOff + bl * kStride, popc(grp));
+        pos0 = read_lane(pos0, leader);
+        if (in_grp) {
+            uint64_t *slot = slot_ptr(q, bl, pos0 + popc(grp & lanemask_lt()));
+            uint32_t spins = 0;
+            while (!cas64(slot, kEmpty, rec)) {
+                if (++spins > q.spin_limit) { ok = false; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        todo &= ~grp;
+    }
+    if (ballot(!ok)) {
+        raise_error(q, kErrOverflow);
+        return false;
+    }
+    return true;
+}
+
+// Window advance (reference :374-382): when the head bucket holds nothing and
+// other buckets do, move the epoch forward by one.  Lane b inspects bucket b.
+__device__ void l2_try_advance(const L2Queue &q, uint32_t base) {
+    const uint32_t lane = lane_id();
+    uint32_t live = 0;
+    if (lane < q.nbuckets)
+        live = ld_relaxed(q.ctl + kTailOff + lane * kStride) -
+               ld_relaxed(q.ctl + kTakenOff + lane * kStride);
+    const uint32_t head_live = read_lane(live, base % q.nbuckets);
+    const uint32_t total = uniform(wave_sum(live));
+    if (head_live == 0 && total > 0 && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
+}
+
+// Pop up to one claim's worth of ready records into `stage[0..got)`.
+// `cl_start`/`cl_mask` carry the wave's claims: lane b holds bucket b.
+__device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
+                           uint64_t *stage, WaveStats &st) {
+    const uint32_t lane = lane_id();
+    const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
+    uint32_t got = 0;
+    bool head_dry = false;
+    for (uint32_t i = 0; i < q.window; ++i) {
+        const uint32_t b = (base + i) % q.nbuckets;
+        uint32_t s = read_lane(cl_start, b);
+        uint64_t mk = read_lane64(cl_mask, b);
+        if (mk == 0) {
+            const uint32_t t = uniform(ld_relaxed(q.ctl + kTailOff + b * kStride));
+            const uint32_t h = uniform(ld_relaxed(q.ctl + kHeadOff + b * kStride));
+            const int32_t avail = (int32_t)(t - h);
+            if (avail <= 0) {
+                if (i == 0) head_dry = true;
+                continue;
+            }
+            const uint32_t k = (uint32_t)avail < q.batch ? (uint32_t)avail : q.batch;
+            uint32_t s0 = 0;
+            if (lane == 0) s0 = fetch_add(q.ctl + kHeadOff + b * kStride, k);
+            s = read_lane(s0, 0);
+            mk = k >= 64 ? ~0ull : ((1ull << k) - 1);
+        }
+        uint64_t v = kEmpty;
+        if ((mk >> lane) & 1) {
+            uint64_t *slot = slot_ptr(q, b, s + lane);
+            v = ld64_relaxed(slot);
+            if (v != kEmpty) v = xchg64(slot, kEmpty);
+        }
+        const uint64_t taken = ballot(v != kEmpty);
+        mk &= ~taken;
+        if (lane == b) {
+            cl_start = s;
+            cl_mask = mk;
+        }
+        if (taken) {
+            if (v != kEmpty) stage[popc(taken & lanemask_lt())] = v;
+            got = popc(taken);
+            if (lane == 0) fetch_add(q.ctl + kTakenOff + b * kStride, got);
+            break;
+        }
+    }
+    if (head_dry) l2_try_advance(q, base);
+    st.v[kStatL2Pop] += got;
+    wave_sync();
+    return got;
+}
+
+// Termination (reference manager warp 0, SS/SSSP/sssp_run.cu:644-661, and
+// get_queue_size, cu_delta_queue.cuh:591-600), with the snapshot order fixed:
+// credits returned are read first, reservations second.  Both counters only
+// grow and reservations are never below credits, so equality of a later
+// reservation sum with an earlier credit count proves that at the earlier
+// instant no record was queued and no wave held unfinished work.
+__device__ bool l2_check_done(const L2Queue &q) {
+    const uint32_t lane = lane_id();
+    const uint32_t done = uniform(ld_acquire(q.ctl + kDoneOff));
+    uint32_t t = 0;
+    if (lane < q.nbuckets) t = ld_relaxed(q.ctl + kTailOff + lane * kStride);
+    const uint32_t reserved = uniform(wave_sum(t));
+    if (reserved == done) {
+        if (lane == 0) st_relaxed(q.ctl + kExitOff, 1u);
+        return true;
+    }
+    return false;
+}
+
+}  // namespace mlmq

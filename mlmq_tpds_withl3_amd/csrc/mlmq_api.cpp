@@ -1,0 +1,581 @@
+// C-ABI host side of the MLMQ solver (see include/mlmq.h for the contract and
+// the reference interface each entry point replaces).
+#include "../../include/mlmq.h"
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+#include "solver_internal.h"
+
+struct mlmq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_reset = nullptr, ev_start = nullptr, ev_stop = nullptr;
+    int cus = 0;
+    int wall_khz = 100000;
+    // graph (device)
+    int32_t n = 0;
+    int64_t m = 0;
+    uint32_t *row = nullptr, *col = nullptr, *wgt = nullptr;
+    bool owns_graph = false;
+    mlmq_graph_info info{};
+    // per-query state
+    uint32_t *dist = nullptr;
+    int32_t dist_n = 0;
+    uint32_t *ctl = nullptr;
+    uint64_t *slots = nullptr;
+    uint64_t slots_count = 0;
+    bool slots_dirty = true;
+    bool solved = false;
+    std::vector<uint32_t> ctl_host;
+    std::string err;
+};
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+int fail(mlmq_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(mlmq_ctx *c, hipError_t e, const char *what) {
+    return fail(c, e == hipErrorOutOfMemory ? MLMQ_E_NOMEM : MLMQ_E_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+void free_graph(mlmq_ctx *c) {
+    if (c->owns_graph) {
+        (void)hipFree(c->row);
+        (void)hipFree(c->col);
+        (void)hipFree(c->wgt);
+    }
+    c->row = c->col = c->wgt = nullptr;
+    c->owns_graph = false;
+    c->n = 0;
+    c->m = 0;
+    c->solved = false;
+}
+
+void compute_info(mlmq_graph_info *gi, int32_t n, int64_t m, const int64_t *row64,
+                  const uint32_t *row32, const int32_t *w) {
+    // Same statistics as graph_info::init_graph_info (core/include/graph_info.h:27-72).
+    auto rowat = [&](int64_t i) -> double { return row64 ? (double)row64[i] : (double)row32[i]; };
+    double sum = 0, mx = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        double d = rowat(i + 1) - rowat(i);
+        sum += d;
+        if (d > mx) mx = d;
+    }
+    double avg = n ? sum / n : 0;
+    double dev = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        double d = rowat(i + 1) - rowat(i) - avg;
+        dev += d * d;
+    }
+    dev = n ? std::sqrt(dev / n) : 0;
+    double ws = 0, wm = 0;
+    for (int64_t e = 0; e < m; ++e) {
+        ws += w[e];
+        if (w[e] > wm) wm = w[e];
+    }
+    double wavg = m ? ws / (double)m : 0;
+    double wdev = 0;
+    for (int64_t e = 0; e < m; ++e) {
+        double d = w[e] - wavg;
+        wdev += d * d;
+    }
+    wdev = m ? std::sqrt(wdev / (double)m) : 0;
+    gi->m = n;
+    gi->nnz = (int32_t)m;
+    gi->avg_nnz = (float)avg;
+    gi->max_nnz = (float)mx;
+    gi->dev_nnz = (float)dev;
+    gi->avg_weight = (float)wavg;
+    gi->max_weight = (float)wm;
+    gi->dev_weight = (float)wdev;
+    gi->diameter = -1.0f;
+}
+
+// Shared validation of a host CSR (0-based starts, row[n] = m).
+int validate_csr(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t *col,
+                 const int32_t *w) {
+    if (n <= 0 || n >= 0x7FFFFFFF) return fail(c, MLMQ_E_INVALID, "vertex count out of range");
+    if (m < 0 || m >= 0x7FFFFFFF) return fail(c, MLMQ_E_INVALID, "edge count out of range");
+    if (!row || (m > 0 && (!col || !w))) return fail(c, MLMQ_E_INVALID, "null CSR array");
+    if (row[0] != 0 || row[n] != m) return fail(c, MLMQ_E_INVALID, "row[0] != 0 or row[n] != m");
+    for (int32_t u = 0; u < n; ++u)
+        if (row[u + 1] < row[u]) return fail(c, MLMQ_E_INVALID, "row offsets not monotone");
+    for (int64_t e = 0; e < m; ++e) {
+        if (col[e] < 0 || col[e] >= n) return fail(c, MLMQ_E_INVALID, "destination out of range");
+        if (w[e] < 0) return fail(c, MLMQ_E_INVALID, "negative edge weight");
+    }
+    return MLMQ_OK;
+}
+
+int upload(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t *col,
+           const int32_t *w) {
+    int rc = validate_csr(c, n, m, row, col, w);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    free_graph(c);
+    std::vector<uint32_t> row32((size_t)n + 1);
+    for (int64_t i = 0; i <= n; ++i) row32[i] = (uint32_t)row[i];
+    const size_t me = (size_t)(m > 0 ? m : 1);
+    HIP_TRY(c, hipMalloc(&c->row, sizeof(uint32_t) * ((size_t)n + 1)));
+    c->owns_graph = true;
+    HIP_TRY(c, hipMalloc(&c->col, sizeof(uint32_t) * me));
+    HIP_TRY(c, hipMalloc(&c->wgt, sizeof(uint32_t) * me));
+    HIP_TRY(c, hipMemcpy(c->row, row32.data(), sizeof(uint32_t) * ((size_t)n + 1),
+                         hipMemcpyHostToDevice));
+    if (m > 0) {
+        HIP_TRY(c, hipMemcpy(c->col, col, sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->wgt, w, sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    }
+    c->n = n;
+    c->m = m;
+    compute_info(&c->info, n, m, row, nullptr, w);
+    return MLMQ_OK;
+}
+
+uint32_t ceil_log2(uint64_t x) {
+    uint32_t k = 0;
+    while ((1ull << k) < x) ++k;
+    return k;
+}
+
+int ensure_dist(mlmq_ctx *c) {
+    if (c->dist && c->dist_n >= c->n) return MLMQ_OK;
+    if (c->dist) (void)hipFree(c->dist);
+    c->dist = nullptr;
+    HIP_TRY(c, hipMalloc(&c->dist, sizeof(uint32_t) * (size_t)c->n));
+    c->dist_n = c->n;
+    return MLMQ_OK;
+}
+
+int ensure_queue(mlmq_ctx *c, uint32_t nbuckets, uint32_t cap_log2) {
+    if (!c->ctl) {
+        HIP_TRY(c, hipMalloc(&c->ctl, sizeof(uint32_t) * mlmq::kCtlWords));
+        c->ctl_host.assign(mlmq::kCtlWords, 0);
+    }
+    const uint64_t need = (uint64_t)nbuckets << cap_log2;
+    if (c->slots && c->slots_count < need) {
+        (void)hipFree(c->slots);
+        c->slots = nullptr;
+        c->slots_count = 0;
+    }
+    if (!c->slots) {
+        HIP_TRY(c, hipMalloc(&c->slots, sizeof(uint64_t) * need));
+        c->slots_count = need;
+        c->slots_dirty = true;
+    }
+    if (c->slots_dirty) {
+        HIP_TRY(c, mlmq::launch_fill_u64(c->slots, c->slots_count, mlmq::kEmpty, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->slots_dirty = false;
+    }
+    return MLMQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mlmq_version(void) { return "mlmq-gfx950 0.1"; }
+
+void mlmq_params_default(mlmq_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->queue_mode = MLMQ_QUEUE_L1SLF_L2DQ;
+    p->delta = 200000;
+    p->work_blocks = 0;
+    p->blocks_per_cu = 0;
+    p->buckets = 16;
+    p->bucket_window = 4;
+    p->l2_batch = 8;
+    p->min_read = 8;
+    p->bucket_capacity = 0;
+    p->timeout_s = 60.0;
+    p->auto_retry = 1;
+}
+
+int mlmq_create(int device, mlmq_ctx **out) {
+    if (!out) return MLMQ_E_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return MLMQ_E_INVALID;
+    mlmq_ctx *c = new (std::nothrow) mlmq_ctx();
+    if (!c) return MLMQ_E_NOMEM;
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_reset);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_stop);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            c->wall_khz = khz;
+    }
+    if (e != hipSuccess) {
+        mlmq_destroy(c);
+        return MLMQ_E_HIP;
+    }
+    *out = c;
+    return MLMQ_OK;
+}
+
+int mlmq_destroy(mlmq_ctx *c) {
+    if (!c) return MLMQ_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    if (c->dist) (void)hipFree(c->dist);
+    if (c->ctl) (void)hipFree(c->ctl);
+    if (c->slots) (void)hipFree(c->slots);
+    if (c->ev_reset) (void)hipEventDestroy(c->ev_reset);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MLMQ_OK;
+}
+
+const char *mlmq_last_error(const mlmq_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mlmq_load_csr(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t *col,
+                  const int32_t *w) {
+    if (!c) return MLMQ_E_INVALID;
+    return upload(c, n, m, row, col, w);
+}
+
+// Galois .gr v1 (SSSP/csr_graph.cu:185-296): header u64 {1, 4, n, m}, n u64
+// row-end offsets, m u32 destinations, 4 pad bytes when m is odd, m int32
+// weights.  Validation as the reference's strict reader
+// (scripts/multigpu/supplement_graph.cpp:16-30).
+int mlmq_load_gr(mlmq_ctx *c, const char *path) {
+    if (!c || !path) return MLMQ_E_INVALID;
+    int fd = open(path, O_RDONLY);
+    if (fd < 0) return fail(c, MLMQ_E_IO, std::string("cannot open ") + path);
+    struct stat sb;
+    if (fstat(fd, &sb) != 0 || sb.st_size < 32) {
+        close(fd);
+        return fail(c, MLMQ_E_IO, "cannot stat or truncated header");
+    }
+    const size_t bytes = (size_t)sb.st_size;
+    void *map = mmap(nullptr, bytes, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (map == MAP_FAILED) return fail(c, MLMQ_E_IO, "mmap failed");
+    const unsigned char *p = (const unsigned char *)map;
+    uint64_t h[4];
+    std::memcpy(h, p, sizeof(h));
+    int rc = MLMQ_OK;
+    if (h[0] != 1 || h[1] != 4 || h[2] == 0 || h[2] >= 0x7FFFFFFFull || h[3] >= 0x7FFFFFFFull) {
+        rc = fail(c, MLMQ_E_INVALID, "unsupported .gr header");
+    } else {
+        const uint64_t n = h[2], m = h[3];
+        const uint64_t need = 32 + 8 * n + 4 * m + 4 * (m & 1) + 4 * m;
+        if (bytes < need) {
+            rc = fail(c, MLMQ_E_INVALID, "truncated .gr file");
+        } else {
+            std::vector<int64_t> row(n + 1);
+            row[0] = 0;
+            const unsigned char *ends = p + 32;
+            for (uint64_t u = 0; u < n; ++u) {
+                uint64_t e;
+                std::memcpy(&e, ends + 8 * u, 8);
+                if (e > m) { rc = fail(c, MLMQ_E_INVALID, "row end beyond m"); break; }
+                row[u + 1] = (int64_t)e;
+            }
+            if (rc == MLMQ_OK) {
+                const int32_t *dst = (const int32_t *)(ends + 8 * n);
+                const int32_t *w = dst + m + (m & 1);
+                rc = upload(c, (int32_t)n, (int64_t)m, row.data(), dst, w);
+            }
+        }
+    }
+    munmap(map, bytes);
+    return rc;
+}
+
+int mlmq_attach_device_csr(mlmq_ctx *c, int32_t n, int64_t m, const int32_t *row_d,
+                           const int32_t *col_d, const int32_t *w_d) {
+    if (!c) return MLMQ_E_INVALID;
+    if (n <= 0 || m < 0 || m >= 0x7FFFFFFF || !row_d || (m > 0 && (!col_d || !w_d)))
+        return fail(c, MLMQ_E_INVALID, "bad device CSR arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // Statistics (and a sanity check of the offsets) need the arrays on the host.
+    std::vector<uint32_t> row((size_t)n + 1);
+    std::vector<int32_t> w((size_t)m);
+    HIP_TRY(c, hipMemcpy(row.data(), row_d, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    if (m > 0) HIP_TRY(c, hipMemcpy(w.data(), w_d, sizeof(int32_t) * (size_t)m, hipMemcpyDeviceToHost));
+    if (row[0] != 0 || (int64_t)row[n] != m) return fail(c, MLMQ_E_INVALID, "row[0] != 0 or row[n] != m");
+    for (int32_t u = 0; u < n; ++u)
+        if (row[u + 1] < row[u]) return fail(c, MLMQ_E_INVALID, "row offsets not monotone");
+    for (int64_t e = 0; e < m; ++e)
+        if (w[e] < 0) return fail(c, MLMQ_E_INVALID, "negative edge weight");
+    free_graph(c);
+    // Non-negative int32 arrays have the same bits as the solver's uint32 view.
+    c->row = (uint32_t *)row_d;
+    c->col = (uint32_t *)col_d;
+    c->wgt = (uint32_t *)w_d;
+    c->owns_graph = false;
+    c->n = n;
+    c->m = m;
+    compute_info(&c->info, n, m, nullptr, row.data(), w.data());
+    return MLMQ_OK;
+}
+
+int mlmq_graph_stats(mlmq_ctx *c, mlmq_graph_info *info) {
+    if (!c || !info) return MLMQ_E_INVALID;
+    if (!c->row) return fail(c, MLMQ_E_NOGRAPH, "no graph loaded");
+    *info = c->info;
+    return MLMQ_OK;
+}
+
+int mlmq_graph_size(const mlmq_ctx *c, int32_t *n, int64_t *m) {
+    if (!c) return MLMQ_E_INVALID;
+    if (n) *n = c->n;
+    if (m) *m = c->m;
+    return c->row ? MLMQ_OK : MLMQ_E_NOGRAPH;
+}
+
+int mlmq_device_shape(mlmq_ctx *c, int32_t *blocks_per_cu_max, int32_t *cus, int32_t *threads) {
+    if (!c) return MLMQ_E_INVALID;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int b = 0;
+    HIP_TRY(c, mlmq::solver_occupancy(true, &b));
+    if (blocks_per_cu_max) *blocks_per_cu_max = b;
+    if (cus) *cus = c->cus;
+    if (threads) *threads = mlmq::kWavesPerBlock * mlmq::kWave;
+    return MLMQ_OK;
+}
+
+int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing *timing) {
+    if (!c) return MLMQ_E_INVALID;
+    if (!c->row) return fail(c, MLMQ_E_NOGRAPH, "no graph loaded");
+    if (src < 0 || src >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
+    mlmq_params p;
+    if (params) p = *params;
+    else mlmq_params_default(&p);
+    const bool slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ;
+    if (!slf && p.queue_mode != MLMQ_QUEUE_L1N_L2DQ)
+        return fail(c, MLMQ_E_UNSUPPORTED, "queue mode not implemented");
+    if (p.delta == 0 || p.delta > 0x7FFFFFFFu) return fail(c, MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
+    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return fail(c, MLMQ_E_INVALID, "buckets must be in [1, 64]");
+    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(c, MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
+    if (p.l2_batch < 1 || p.l2_batch > 64) return fail(c, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
+    if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
+    if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
+        return fail(c, MLMQ_E_INVALID, "negative size parameter");
+    HIP_TRY(c, hipSetDevice(c->device));
+
+    int occ = 0;
+    HIP_TRY(c, mlmq::solver_occupancy(slf, &occ));
+    if (occ < 1) return fail(c, MLMQ_E_HIP, "solver kernel cannot be resident");
+    int blocks = p.work_blocks;
+    if (blocks == 0) {
+        int per_cu = p.blocks_per_cu ? p.blocks_per_cu : 2;
+        if (per_cu > occ) per_cu = occ;
+        blocks = per_cu * c->cus;
+    }
+    if (blocks > occ * c->cus) blocks = occ * c->cus;
+
+    uint64_t cap = (uint64_t)p.bucket_capacity;
+    if (cap == 0) cap = (uint64_t)c->n > 8192 ? (uint64_t)c->n : 8192;
+    uint32_t cap_log2 = ceil_log2(cap);
+    if (cap_log2 > 31) return fail(c, MLMQ_E_INVALID, "bucket capacity too large");
+
+    int rc = ensure_dist(c);
+    if (rc) return rc;
+
+    const double timeout_s = p.timeout_s > 0 ? p.timeout_s : 60.0;
+    uint64_t retries = 0;
+    for (;;) {
+        rc = ensure_queue(c, (uint32_t)p.buckets, cap_log2);
+        if (rc) return rc;
+        const auto q0 = clk::now();
+        HIP_TRY(c, hipEventRecord(c->ev_reset, c->stream));
+        HIP_TRY(c, mlmq::launch_reset(c->dist, (uint32_t)c->n, (uint32_t)src, c->ctl, c->slots, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const auto q1 = clk::now();
+
+        mlmq::SolverLaunch L;
+        L.row = c->row;
+        L.col = c->col;
+        L.wgt = c->wgt;
+        L.dist = c->dist;
+        L.slots = c->slots;
+        L.ctl = c->ctl;
+        L.n = (uint32_t)c->n;
+        L.nbuckets = (uint32_t)p.buckets;
+        L.window = (uint32_t)p.bucket_window;
+        L.cap_log2 = cap_log2;
+        L.delta = p.delta;
+        L.batch = (uint32_t)p.l2_batch;
+        L.spin_limit = 1u << 22;
+        L.min_read = (uint32_t)p.min_read;
+        L.max_sleep = 16;
+        L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
+        L.blocks = blocks;
+        L.slf = slf;
+
+        const auto s0 = clk::now();
+        HIP_TRY(c, hipEventRecord(c->ev_start, c->stream));
+        HIP_TRY(c, mlmq::launch_solver(L, c->stream));
+        HIP_TRY(c, hipEventRecord(c->ev_stop, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const auto s1 = clk::now();
+
+        HIP_TRY(c, hipMemcpy(c->ctl_host.data(), c->ctl, sizeof(uint32_t) * mlmq::kCtlWords,
+                             hipMemcpyDeviceToHost));
+        const uint32_t *h = c->ctl_host.data();
+        const uint32_t derr = h[mlmq::kErrOff];
+        uint32_t reserved = 0, taken = 0;
+        for (int b = 0; b < p.buckets; ++b) {
+            reserved += h[mlmq::kTailOff + b * mlmq::kStride];
+            taken += h[mlmq::kTakenOff + b * mlmq::kStride];
+        }
+        const uint32_t done = h[mlmq::kDoneOff];
+        if (derr != mlmq::kErrNone) c->slots_dirty = true;
+        if (derr == mlmq::kErrOverflow) {
+            if (p.auto_retry && retries < 4 && cap_log2 < 31) {
+                ++retries;
+                ++cap_log2;
+                continue;
+            }
+            return fail(c, MLMQ_E_QUEUE_OVERFLOW, "L2 bucket ring overflow; raise bucket_capacity");
+        }
+        if (derr == mlmq::kErrTimeout) return fail(c, MLMQ_E_TIMEOUT, "device watchdog expired");
+        if (derr != mlmq::kErrNone) return fail(c, MLMQ_E_INTERNAL, "unknown device error");
+        if (h[mlmq::kExitOff] != 1 || reserved != taken || reserved != done) {
+            c->slots_dirty = true;
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "queue accounting mismatch: reserved=%u taken=%u done=%u exit=%u",
+                          reserved, taken, done, h[mlmq::kExitOff]);
+            return fail(c, MLMQ_E_INTERNAL, buf);
+        }
+        c->solved = true;
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            float dev_ms = 0, reset_ms = 0;
+            (void)hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_stop);
+            (void)hipEventElapsedTime(&reset_ms, c->ev_reset, c->ev_start);
+            timing->solve_ms = ms_since(s0, s1);
+            timing->device_ms = dev_ms;
+            timing->reset_ms = ms_since(q0, q1);
+            timing->query_wall_ms = ms_since(q0, s1);
+            timing->work_blocks = blocks;
+            timing->waves = blocks * mlmq::kWavesPerBlock;
+            timing->bucket_capacity = (int64_t)1 << cap_log2;
+            const uint64_t *st = reinterpret_cast<const uint64_t *>(h + mlmq::kStatsOff);
+            timing->l2_pushes = st[mlmq::kStatL2Push] + 1;  // + the seed
+            timing->l2_pops = st[mlmq::kStatL2Pop];
+            timing->relaxations = st[mlmq::kStatRelax];
+            timing->atomics = st[mlmq::kStatAtomic];
+            timing->improvements = st[mlmq::kStatImprove];
+            timing->stale = st[mlmq::kStatStale];
+            timing->batches = st[mlmq::kStatBatches];
+            timing->idle_polls = st[mlmq::kStatIdle];
+            timing->retries = retries;
+        }
+        return MLMQ_OK;
+    }
+}
+
+int mlmq_copy_dist(mlmq_ctx *c, int32_t *host_out) {
+    if (!c || !host_out) return MLMQ_E_INVALID;
+    if (!c->solved) return fail(c, MLMQ_E_NOGRAPH, "no completed solve");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(host_out, c->dist, sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToHost));
+    return MLMQ_OK;
+}
+
+int mlmq_dist_device_ptr(mlmq_ctx *c, const int32_t **out) {
+    if (!c || !out) return MLMQ_E_INVALID;
+    if (!c->solved) return fail(c, MLMQ_E_NOGRAPH, "no completed solve");
+    *out = (const int32_t *)c->dist;
+    return MLMQ_OK;
+}
+
+// ---- reference driver ABI: one implicit context on device 0 ----
+static mlmq_ctx *g_ref_ctx = nullptr;
+
+static int env_int(const char *name, long lo, long hi, long fallback, long *out) {
+    const char *s = std::getenv(name);
+    if (!s) {
+        *out = fallback;
+        return MLMQ_OK;
+    }
+    char *end = nullptr;
+    long v = std::strtol(s, &end, 10);
+    if (end == s || *end || v < lo || v > hi) return MLMQ_E_INVALID;
+    *out = v;
+    return MLMQ_OK;
+}
+
+int sssp_init(int m, int nnz, int *RowPtr_d, int *ColIdx_d, int *w_d) {
+    if (!g_ref_ctx) {
+        int rc = mlmq_create(0, &g_ref_ctx);
+        if (rc) return rc;
+    }
+    return mlmq_attach_device_csr(g_ref_ctx, m, nnz, RowPtr_d, ColIdx_d, w_d);
+}
+
+int sssp_run_adaptive(int src, mlmq_graph_info info) {
+    (void)info;  // the adaptive policy is disabled on the reference path too
+    if (!g_ref_ctx) return MLMQ_E_NOGRAPH;
+    mlmq_params p;
+    mlmq_params_default(&p);
+    long v = 0;
+    if (env_int("BENCH_DELTA", 1, 0x7FFFFFFF, 200000, &v)) return MLMQ_E_INVALID;
+    p.delta = (uint32_t)v;
+    if (env_int("MLMQ_WORK_BLOCKS", 0, 1 << 20, 0, &v)) return MLMQ_E_INVALID;
+    p.work_blocks = (int32_t)v;
+    return mlmq_solve(g_ref_ctx, src, &p, nullptr);
+}
+
+int sssp_cp_data(int **node_data_h) {
+    if (!node_data_h || !g_ref_ctx) return MLMQ_E_INVALID;
+    int32_t *buf = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)g_ref_ctx->n);
+    if (!buf) return MLMQ_E_NOMEM;
+    int rc = mlmq_copy_dist(g_ref_ctx, buf);
+    if (rc) {
+        std::free(buf);
+        return rc;
+    }
+    *node_data_h = buf;
+    return MLMQ_OK;
+}
+
+int sssp_release(void) {
+    int rc = mlmq_destroy(g_ref_ctx);
+    g_ref_ctx = nullptr;
+    return rc;
+}
+
+}  // extern "C"

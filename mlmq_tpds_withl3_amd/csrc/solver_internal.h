@@ -1,0 +1,41 @@
+// Internal interface between the C-ABI host code (mlmq_api.cpp) and the HIP
+// kernels (sssp_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlmq {
+
+constexpr int kWavesPerBlock = 4;   // 256-thread persistent workgroups
+
+struct SolverLaunch {
+    const uint32_t *row;
+    const uint32_t *col;
+    const uint32_t *wgt;
+    uint32_t *dist;
+    uint64_t *slots;
+    uint32_t *ctl;
+    uint32_t n;
+    uint32_t nbuckets;
+    uint32_t window;
+    uint32_t cap_log2;
+    uint32_t delta;
+    uint32_t batch;
+    uint32_t spin_limit;
+    uint32_t min_read;
+    uint32_t max_sleep;
+    uint64_t timeout_ticks;
+    int blocks;
+    bool slf;
+};
+
+hipError_t launch_reset(uint32_t *dist, uint32_t n, uint32_t src, uint32_t *ctl, uint64_t *slots,
+                        hipStream_t s);
+hipError_t launch_fill_u64(uint64_t *p, uint64_t count, uint64_t value, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);
+hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);
+hipError_t solver_occupancy(bool slf, int *blocks_per_cu);
+hipError_t launch_solver(const SolverLaunch &L, hipStream_t s);
+
+}  // namespace mlmq

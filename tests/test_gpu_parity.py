@@ -1,0 +1,157 @@
+"""Bit-exact parity of the HIP solver with the CPU oracle (needs an MI355X).
+
+Every distance of every query must equal the oracle's (int32, INT_MAX for
+unreachable).  Cases follow the reference's checks: the 2052-vertex fixture
+with its three golden sources and the same-process source switch 0 -> 1030
+-> 2050 -> 0 (scripts/multigpu/run_supplement_checks.py:27-53), plus
+road-like, grid and power-law graphs, parameter corners and error paths.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from graphs_util import random_csr, grid_csr
+from mlmq_tpds_withl3_amd import Solver, HostGraph, MLMQError, make_workload
+from mlmq_tpds_withl3_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TIMEOUT = 20.0   # device watchdog per solve in tests
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = Solver(0)
+    yield s
+    s.close()
+
+
+def solve_check(solver, row, col, w, src, load=True, **params):
+    if load:
+        solver.load_csr(row, col, w)
+    params.setdefault("timeout_s", TIMEOUT)
+    t = solver.solve(src, **params)
+    got = solver.dist()
+    ref, work, reached = O.dijkstra(row, col, w, src)
+    bad = np.flatnonzero(got != ref)
+    assert bad.size == 0, (f"{bad.size} mismatches, first v={bad[0]} gpu={got[bad[0]]} "
+                           f"cpu={ref[bad[0]]} params={params}")
+    return t, work, reached
+
+
+def test_fixture_golden_and_source_switch(solver):
+    say("fixture golden + source switch")
+    solver.load_gr(os.path.join(GOLDEN, "fixture.gr"))
+    for src in (0, 1030, 2050, 0):
+        solver.solve(src, timeout_s=TIMEOUT)
+        exp = np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), dtype=np.int32)
+        assert np.array_equal(solver.dist(), exp), f"source {src}"
+
+
+@pytest.mark.parametrize("queue", ["L1SLF_L2DQ", "L1N_L2DQ"])
+@pytest.mark.parametrize("seed", range(6))
+def test_random_graphs(solver, seed, queue):
+    say(f"random seed={seed} queue={queue}")
+    n = 200 + 700 * seed
+    row, col, w = random_csr(n, 4 * n, seed, wmax=1000, zero_frac=0.1)
+    for src in (0, n - 1):
+        solve_check(solver, row, col, w, src, queue=queue, delta=300)
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(),
+    dict(delta=500),                                     # many buckets, window advance
+    dict(delta=50, buckets=4, bucket_window=1),
+    dict(buckets=1, bucket_window=1, l2_batch=1, min_read=1),
+    dict(l2_batch=64, min_read=64, delta=2000),
+    dict(work_blocks=1),                                 # a single workgroup
+    dict(work_blocks=3, delta=1000),
+    dict(blocks_per_cu=4, delta=5000),
+    dict(queue="L1N_L2DQ", delta=2000),
+])
+def test_grid_param_corners(solver, cfg):
+    say(f"grid 120x150 {cfg}")
+    row, col, w = grid_csr(120, 150, 7)
+    solve_check(solver, row, col, w, 0, **cfg)
+
+
+def test_zero_weight_components(solver):
+    say("zero weights")
+    row, col, w = random_csr(5000, 15000, 99, wmax=3, zero_frac=0.6)
+    for src in (1, 2500):
+        solve_check(solver, row, col, w, src, delta=1)
+
+
+def test_road264k(solver):
+    say("road264k generate")
+    g, src, spec = make_workload("road264k")
+    row, col, w = g.arrays()
+    for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ")):
+        say(f"road264k {cfg}")
+        t, work, reached = solve_check(solver, row, col, w, src, **cfg)
+        assert t["relaxations"] >= work
+
+
+def test_rmat_small(solver):
+    say("rmat scale 16")
+    g = HostGraph.rmat(16, 16, seed=5)
+    row, col, w = g.arrays()
+    src = g.max_degree_vertex()
+    for cfg in (dict(), dict(delta=100)):
+        solve_check(solver, row, col, w, src, **cfg)
+
+
+def test_road6m_full(solver):
+    say("road6m generate")
+    g, src, spec = make_workload("road6m")
+    row, col, w = g.arrays()
+    say(f"road6m solve n={g.n} m={g.m} src={src}")
+    t, work, reached = solve_check(solver, row, col, w, src)
+    say(f"road6m solve_ms={t['solve_ms']:.3f}")
+    assert reached == g.n
+
+
+def test_reference_driver_abi():
+    torch = pytest.importorskip("torch")
+    say("reference ABI sssp_init/run/cp_data")
+    row, col, w = O.read_gr(os.path.join(GOLDEN, "fixture.gr"))
+    dev = torch.device("cuda:0")
+    r = torch.tensor(row.astype(np.int32), device=dev)
+    c = torch.tensor(col, device=dev)
+    ww = torch.tensor(w, device=dev)
+    torch.cuda.synchronize()
+    S.sssp_init(len(row) - 1, len(col), r.data_ptr(), c.data_ptr(), ww.data_ptr())
+    for src in (1030, 0):
+        S.sssp_run_adaptive(src)
+        got = S.sssp_cp_data()
+        assert np.array_equal(got, np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), np.int32))
+    S.sssp_release()
+
+
+def test_errors_and_recovery(solver):
+    say("error paths")
+    row, col, w = grid_csr(200, 200, 3)
+    solver.load_csr(row, col, w)
+    with pytest.raises(MLMQError) as e:
+        solver.solve(-1)
+    assert e.value.code == -1
+    with pytest.raises(MLMQError) as e:
+        solver.solve(0, buckets=0)
+    assert e.value.code == -1
+    with pytest.raises(MLMQError) as e:
+        solver.solve(0, queue_mode=0)
+    assert e.value.code == -9
+    # watchdog forced from the host: every wave sees the deadline passed and exits
+    with pytest.raises(MLMQError) as e:
+        solver.solve(0, timeout_s=1e-7, delta=100)
+    assert e.value.code == -7
+    # the next query still runs on a clean workspace
+    solve_check(solver, row, col, w, 0, load=False)

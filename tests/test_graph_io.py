@@ -1,0 +1,152 @@
+"""Graph formats and synthetic inputs of the product library (CPU only)."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import make_fixture
+from oracle import oracle as O
+from mlmq_tpds_withl3_amd import HostGraph
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+FIXTURE_SHA = "24b038d8d69316667b28647d457aa0096bc7c756789a3b539255151617501821"
+
+
+def test_fixture_regenerates_byte_exact():
+    blob = make_fixture.fixture_bytes()
+    assert hashlib.sha256(blob).hexdigest() == FIXTURE_SHA
+    assert blob == open(os.path.join(GOLDEN, "fixture.gr"), "rb").read()
+
+
+def test_product_reader_matches_numpy_reader():
+    path = os.path.join(GOLDEN, "fixture.gr")
+    g = HostGraph.read_gr(path)
+    row, col, w = g.arrays()
+    r2, c2, w2 = O.read_gr(path)
+    assert g.n == 2052 and g.m == 5638
+    assert np.array_equal(row, r2) and np.array_equal(col, c2) and np.array_equal(w, w2)
+
+
+def test_product_writer_round_trip(tmp_path):
+    g = HostGraph.read_gr(os.path.join(GOLDEN, "fixture.gr"))
+    out = str(tmp_path / "copy.gr")
+    g.write_gr(out)
+    assert open(out, "rb").read() == open(os.path.join(GOLDEN, "fixture.gr"), "rb").read()
+
+
+def test_odd_edge_count_padding(tmp_path):
+    row = np.array([0, 2, 3], dtype=np.int64)
+    col = np.array([1, 0, 0], dtype=np.int32)
+    w = np.array([5, 0, 7], dtype=np.int32)
+    g = HostGraph.from_csr(row, col, w)
+    p = str(tmp_path / "odd.gr")
+    g.write_gr(p)
+    assert os.path.getsize(p) == 32 + 8 * 2 + 4 * 3 + 4 + 4 * 3
+    r2, c2, w2 = O.read_gr(p)
+    assert np.array_equal(r2, row) and np.array_equal(c2, col) and np.array_equal(w2, w)
+
+
+@pytest.mark.parametrize("mutate", ["truncate", "version", "edge_size", "dst_range", "neg_weight"])
+def test_reader_rejects_malformed(tmp_path, mutate):
+    img = bytearray(open(os.path.join(GOLDEN, "fixture.gr"), "rb").read())
+    n, m = 2052, 5638
+    dst_off = 32 + 8 * n
+    w_off = dst_off + 4 * m
+    if mutate == "truncate":
+        img = img[:-8]
+    elif mutate == "version":
+        img[0] = 2
+    elif mutate == "edge_size":
+        img[8] = 8
+    elif mutate == "dst_range":
+        img[dst_off:dst_off + 4] = (5000).to_bytes(4, "little")
+    elif mutate == "neg_weight":
+        img[w_off:w_off + 4] = (-3).to_bytes(4, "little", signed=True)
+    p = str(tmp_path / "bad.gr")
+    open(p, "wb").write(bytes(img))
+    with pytest.raises(Exception):
+        HostGraph.read_gr(p)
+
+
+def _brute_knn_pairs(xy, k):
+    n = len(xy)
+    d2 = ((xy[:, None, :] - xy[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    pairs = set()
+    for i in range(n):
+        order = np.lexsort((np.arange(n), d2[i]))[:k]
+        for j in order:
+            pairs.add((min(i, int(j)), max(i, int(j))))
+    return pairs
+
+
+def test_knn_generator_structure():
+    g = HostGraph.knn(3000, 3, seed=11, weight_scale=1e5)
+    row, col, w = g.arrays()
+    n = g.n
+    src = np.repeat(np.arange(n), np.diff(row))
+    assert np.all(col != src)                                   # no self loops
+    key = src.astype(np.int64) * n + col
+    assert len(np.unique(key)) == len(key)                      # no duplicates
+    rev = col.astype(np.int64) * n + src
+    assert set(key.tolist()) == set(rev.tolist())               # symmetric
+    assert np.all(np.diff(row) >= 3)                            # every vertex keeps its 3 NN
+    assert w.min() >= 1
+    # symmetric weights
+    wmap = dict(zip(key.tolist(), w.tolist()))
+    assert all(wmap[a * n + b] == wmap[b * n + a] for a, b in zip(src[:500], col[:500]))
+
+
+def test_knn_generator_deterministic_and_morton():
+    a = HostGraph.knn(5000, 3, seed=5, weight_scale=-50_000.0, morton=True)
+    b = HostGraph.knn(5000, 3, seed=5, weight_scale=-50_000.0, morton=True)
+    for x, y in zip(a.arrays(), b.arrays()):
+        assert np.array_equal(x, y)
+    assert a.arrays()[2].max() == 50_000                        # longest edge scaled to 50k
+    c = HostGraph.knn(5000, 3, seed=6, weight_scale=-50_000.0, morton=True)
+    assert not np.array_equal(a.arrays()[1], c.arrays()[1])
+    # Morton order keeps neighbours close in id space on average
+    row, col, _ = a.arrays()
+    src = np.repeat(np.arange(a.n), np.diff(row))
+    assert np.median(np.abs(col - src)) < a.n / 50
+
+
+def test_knn_matches_brute_force():
+    g = HostGraph.knn(400, 3, seed=3, weight_scale=1e6)
+    xy = g.coords()
+    row, col, w = g.arrays()
+    pairs = set()
+    for u in range(g.n):
+        for v in col[row[u]:row[u + 1]]:
+            pairs.add((min(u, int(v)), max(u, int(v))))
+    assert pairs == _brute_knn_pairs(xy, 3)
+    # weight = max(1, round(len * scale))
+    u = np.repeat(np.arange(g.n), np.diff(row))
+    length = np.sqrt(((xy[u] - xy[col]) ** 2).sum(1))
+    assert np.array_equal(w, np.maximum(1, np.round(length * 1e6)).astype(np.int32))
+
+
+def test_centroid_vertex():
+    g = HostGraph.knn(2000, 3, seed=4, weight_scale=1e5)
+    xy = g.coords()
+    c = xy.mean(0)
+    assert g.centroid_vertex() == int(np.argmin(((xy - c) ** 2).sum(1)))
+
+
+def test_rmat_generator():
+    g = HostGraph.rmat(12, 8, seed=9, wmin=1, wmax=1000)
+    row, col, w = g.arrays()
+    n = g.n
+    assert n == 4096
+    src = np.repeat(np.arange(n), np.diff(row))
+    assert np.all(col != src)
+    key = src.astype(np.int64) * n + col
+    assert len(np.unique(key)) == len(key)
+    assert set(key.tolist()) == set((col.astype(np.int64) * n + src).tolist())
+    assert w.min() >= 1 and w.max() <= 1000
+    deg = np.diff(row)
+    assert g.max_degree_vertex() == int(np.argmax(deg))
+    assert deg.max() > 20 * deg.mean()                          # skewed (power-law-like)
+    h = HostGraph.rmat(12, 8, seed=9, wmin=1, wmax=1000)
+    assert all(np.array_equal(x, y) for x, y in zip(g.arrays(), h.arrays()))
