@@ -75,7 +75,8 @@ typedef struct mlmq_params {
     int64_t  bucket_capacity; /* ring slots per bucket (rounded up to 2^k); 0 = auto */
     double   timeout_s;       /* device watchdog; <= 0 means 60 s */
     int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
-    int32_t  reserved[7];
+    int32_t  profile;         /* 1: per-phase clock build of the kernel (mlmq_timing.phase_cycles) */
+    int32_t  reserved[6];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -96,6 +97,10 @@ typedef struct mlmq_timing {
     uint64_t batches;         /* processed record batches */
     uint64_t idle_polls;      /* idle iterations of all waves */
     uint64_t retries;         /* overflow retries performed by mlmq_solve */
+    /* params.profile = 1 only: shader-clock cycles summed over waves spent in
+     * L1 pop, L2 pop, CSR-range fetch, edge relaxation, push/flush, idle,
+     * total, credit return */
+    uint64_t phase_cycles[8];
     uint64_t reserved[6];
 } mlmq_timing;
 

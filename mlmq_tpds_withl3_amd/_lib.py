@@ -49,7 +49,8 @@ class Params(ctypes.Structure):
         ("buckets", ctypes.c_int32), ("bucket_window", ctypes.c_int32),
         ("l2_batch", ctypes.c_int32), ("min_read", ctypes.c_int32),
         ("bucket_capacity", ctypes.c_int64), ("timeout_s", ctypes.c_double),
-        ("auto_retry", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("auto_retry", ctypes.c_int32), ("profile", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -63,11 +64,15 @@ class Timing(ctypes.Structure):
         ("relaxations", ctypes.c_uint64), ("atomics", ctypes.c_uint64),
         ("improvements", ctypes.c_uint64), ("stale", ctypes.c_uint64),
         ("batches", ctypes.c_uint64), ("idle_polls", ctypes.c_uint64),
-        ("retries", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 6),
+        ("retries", ctypes.c_uint64), ("phase_cycles", ctypes.c_uint64 * 8),
+        ("reserved", ctypes.c_uint64 * 6),
     ]
+    PHASES = ("l1_pop", "l2_pop", "row_fetch", "edges", "flush", "idle", "total", "credit")
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "phase_cycles")}
+        d["phase_cycles"] = dict(zip(self.PHASES, list(self.phase_cycles)))
+        return d
 
 
 class GraphInfo(ctypes.Structure):

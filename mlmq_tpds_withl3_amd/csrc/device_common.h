@@ -33,7 +33,9 @@ constexpr uint32_t kCtlWords = kStatsOff + 2 * kNumStats + kStride;
 enum DeviceError : uint32_t { kErrNone = 0, kErrOverflow = 1, kErrTimeout = 2 };
 enum StatIndex : uint32_t {
     kStatL2Push = 0, kStatL2Pop, kStatRelax, kStatAtomic, kStatImprove,
-    kStatStale, kStatBatches, kStatIdle
+    kStatStale, kStatBatches, kStatIdle,
+    // per-phase clock totals (profiling build of the kernel only)
+    kClkL1Pop, kClkL2Pop, kClkRow, kClkEdges, kClkFlush, kClkIdle, kClkTotal, kClkCredit
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

@@ -23,6 +23,9 @@
 //    stops, instead of silently overwriting.
 //  * A claim is (start, 64-bit pending mask) held in lane b of the wave for
 //    bucket b, so a wave keeps claims on all buckets in two registers.
+//  * A reader takes one snapshot of every bucket's counters (lane b loads
+//    bucket b's tail/head/taken, all lanes the epoch) in a single round trip,
+//    then decides locally; no per-bucket sequence of loads.
 //  * Counters live on separate 128-byte lines; all are agent-scope atomics.
 #pragma once
 
@@ -97,51 +100,42 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
     return true;
 }
 
-// Window advance (reference :374-382): when the head bucket holds nothing and
-// other buckets do, move the epoch forward by one.  Lane b inspects bucket b.
-__device__ void l2_try_advance(const L2Queue &q, uint32_t base) {
-    const uint32_t lane = lane_id();
-    uint32_t live = 0;
-    if (lane < q.nbuckets)
-        live = ld_relaxed(q.ctl + kTailOff + lane * kStride) -
-               ld_relaxed(q.ctl + kTakenOff + lane * kStride);
-    const uint32_t head_live = read_lane(live, base % q.nbuckets);
-    const uint32_t total = uniform(wave_sum(live));
-    if (head_live == 0 && total > 0 && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
-}
-
 // Pop up to one claim's worth of ready records into `stage[0..got)`.
 // `cl_start`/`cl_mask` carry the wave's claims: lane b holds bucket b.
 __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
                            uint64_t *stage, WaveStats &st) {
     const uint32_t lane = lane_id();
+    const bool mine = lane < q.nbuckets;
+    // one round trip: the epoch and every bucket's counters
     const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
+    uint32_t t = 0, h = 0, k = 0;
+    if (mine) {
+        t = ld_relaxed(q.ctl + kTailOff + lane * kStride);
+        h = ld_relaxed(q.ctl + kHeadOff + lane * kStride);
+        k = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
+    }
+    const bool unclaimed = mine && (int32_t)(t - h) > 0;
+    const uint64_t avail_mask = ballot(unclaimed);
+    const uint64_t claim_mask = ballot(mine && cl_mask != 0);
     uint32_t got = 0;
-    bool head_dry = false;
-    for (uint32_t i = 0; i < q.window; ++i) {
+    for (uint32_t i = 0; i < q.window && got == 0; ++i) {
         const uint32_t b = (base + i) % q.nbuckets;
+        const bool has_claim = (claim_mask >> b) & 1;
+        if (!has_claim && !((avail_mask >> b) & 1)) continue;
         uint32_t s = read_lane(cl_start, b);
         uint64_t mk = read_lane64(cl_mask, b);
-        if (mk == 0) {
-            const uint32_t t = uniform(ld_relaxed(q.ctl + kTailOff + b * kStride));
-            const uint32_t h = uniform(ld_relaxed(q.ctl + kHeadOff + b * kStride));
-            const int32_t avail = (int32_t)(t - h);
-            if (avail <= 0) {
-                if (i == 0) head_dry = true;
-                continue;
-            }
-            const uint32_t k = (uint32_t)avail < q.batch ? (uint32_t)avail : q.batch;
+        if (!has_claim) {
+            const uint32_t avail = read_lane(t, b) - read_lane(h, b);
+            const uint32_t kk = avail < q.batch ? avail : q.batch;
             uint32_t s0 = 0;
-            if (lane == 0) s0 = fetch_add(q.ctl + kHeadOff + b * kStride, k);
+            if (lane == 0) s0 = fetch_add(q.ctl + kHeadOff + b * kStride, kk);
             s = read_lane(s0, 0);
-            mk = k >= 64 ? ~0ull : ((1ull << k) - 1);
+            mk = kk >= 64 ? ~0ull : ((1ull << kk) - 1);
         }
+        // Take every claimed slot that holds a record; an empty slot is left
+        // empty by the exchange (a writer's later CAS still succeeds).
         uint64_t v = kEmpty;
-        if ((mk >> lane) & 1) {
-            uint64_t *slot = slot_ptr(q, b, s + lane);
-            v = ld64_relaxed(slot);
-            if (v != kEmpty) v = xchg64(slot, kEmpty);
-        }
+        if ((mk >> lane) & 1) v = xchg64(slot_ptr(q, b, s + lane), kEmpty);
         const uint64_t taken = ballot(v != kEmpty);
         mk &= ~taken;
         if (lane == b) {
@@ -152,10 +146,16 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
             if (v != kEmpty) stage[popc(taken & lanemask_lt())] = v;
             got = popc(taken);
             if (lane == 0) fetch_add(q.ctl + kTakenOff + b * kStride, got);
-            break;
         }
     }
-    if (head_dry) l2_try_advance(q, base);
+    // Window advance: head bucket drained (every reserved record taken) while
+    // other buckets still hold records.
+    const uint32_t b0 = base % q.nbuckets;
+    const uint32_t live = mine ? t - k : 0;
+    if (read_lane(live, b0) == 0 && !((claim_mask >> b0) & 1)) {
+        const uint32_t total = uniform(wave_sum(live));
+        if (total > 0 && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
+    }
     st.v[kStatL2Pop] += got;
     wave_sync();
     return got;

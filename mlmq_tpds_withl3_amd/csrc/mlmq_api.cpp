@@ -443,6 +443,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
         L.blocks = blocks;
         L.slf = slf;
+        L.profile = p.profile != 0;
 
         const auto s0 = clk::now();
         HIP_TRY(c, hipEventRecord(c->ev_start, c->stream));
@@ -502,6 +503,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
             timing->batches = st[mlmq::kStatBatches];
             timing->idle_polls = st[mlmq::kStatIdle];
             timing->retries = retries;
+            for (int i = 0; i < 8; ++i) timing->phase_cycles[i] = st[mlmq::kClkL1Pop + i];
         }
         return MLMQ_OK;
     }

@@ -28,6 +28,7 @@ struct SolverLaunch {
     uint64_t timeout_ticks;
     int blocks;
     bool slf;
+    bool profile;   // per-phase clock build of the kernel
 };
 
 hipError_t launch_reset(uint32_t *dist, uint32_t n, uint32_t src, uint32_t *ctl, uint64_t *slots,

@@ -35,50 +35,80 @@ struct SolveArgs {
     uint64_t timeout_ticks;
 };
 
-// Flush the newest `cnt` (<= 64) pending records from `out` to L1 (or L2).
+template <bool kProf>
+struct PhaseClock {
+    uint64_t t;
+    __device__ __forceinline__ void start() {
+        if (kProf) t = clock64();
+    }
+    __device__ __forceinline__ void lap(WaveStats &st, uint32_t slot) {
+        if (kProf) {
+            const uint64_t now = clock64();
+            st.v[slot] += now - t;
+            t = now;
+        }
+    }
+};
+
+// Flush the newest `cnt` (<= 64) pending entries of `out` to L1 (or L2).
 template <bool kSLF>
-__device__ __forceinline__ bool flush_out(const uint64_t *out, uint32_t &out_n, uint32_t cnt,
-                                          uint64_t *ring, L1State &l1, const L2Queue &q,
+__device__ __forceinline__ bool flush_out(const L1Ring &out, uint32_t &out_n, uint32_t cnt,
+                                          const L1Ring &ring, L1State &l1, const L2Queue &q,
                                           WaveStats &st) {
     const uint32_t lane = lane_id();
-    uint64_t r = kEmpty;
-    if (lane < cnt) r = out[out_n - cnt + lane];
+    uint64_t r = kEmpty, w = kNoRow;
+    if (lane < cnt) {
+        r = out.rec[out_n - cnt + lane];
+        w = out.row[out_n - cnt + lane];
+    }
     wave_sync();
     out_n -= cnt;
-    if (kSLF) return l1_push(ring, l1, r, cnt, q, st);
+    if (kSLF) return l1_push(ring, l1, r, w, cnt, q, st);
     return l2_push(q, r, lane < cnt, st);
 }
 
-// Relax all out-edges of the records in lanes [0, nin) (simple_process).
-// A record is skipped when its distance is above the vertex's current one
-// (reference filter, SS/SSSP/sssp_run.cu:174: process on <=).  Edge slots are
-// spread over the 64 lanes by an exclusive scan of degrees; each lane finds
-// its owner by a binary search over the scan held in lanes.  A lane pushes
-// (dst, nd) only if its atomicMin strictly lowered dist[dst] (:211-243).
-template <bool kSLF>
-__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, uint64_t *out,
-                              uint64_t *ring, L1State &l1, WaveStats &st) {
+// Relax all out-edges of the records in lanes [0, nin) (simple_process,
+// SS/SSSP/sssp_run.cu:143-413).  A record is skipped when its distance is
+// above the vertex's current one (reference filter :174, process on <=).
+// Edge slots are spread over the 64 lanes by an exclusive scan of degrees;
+// each lane finds its owner by a binary search over the scan held in lanes.
+// A lane pushes (dst, nd) only if its atomicMin strictly lowered dist[dst]
+// (:211-243).
+//
+// Latency shape (one hop = two dependent round trips): the stale check of
+// the batch is in flight together with the first chunk's col/weight loads,
+// and each relaxation's atomicMin is in flight together with the loads of
+// dst's CSR range, which travels with the pushed record.
+template <bool kSLF, bool kProf>
+__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint64_t rowi, uint32_t nin,
+                              const L1Ring &out, const L1Ring &ring, L1State &l1, WaveStats &st,
+                              PhaseClock<kProf> &clk) {
     const uint32_t lane = lane_id();
-    bool valid = lane < nin;
+    const bool valid = lane < nin;
     const uint32_t v = rec_vertex(rec);
     const uint32_t d = rec_dist(rec);
-    uint32_t start = 0, deg = 0;
-    if (valid) {
-        const uint32_t cur = ld_relaxed(a.dist + v);
-        if (d > cur) valid = false;
+    uint32_t start = (uint32_t)rowi, deg = (uint32_t)(rowi >> 32);
+    // Records that came through L2 carry no CSR range.
+    const bool need_row = valid && rowi == kNoRow;
+    if (ballot(need_row)) {
+        if (need_row) {
+            start = a.row[v];
+            deg = a.row[v + 1] - start;
+        }
+        clk.lap(st, kClkRow);
     }
-    st.v[kStatStale] += popc(ballot(lane < nin && !valid));
-    if (valid) {
-        start = a.row[v];
-        deg = a.row[v + 1] - start;
-    }
+    if (!valid) deg = 0;
+    uint32_t cur = 0;
+    if (valid) cur = ld_relaxed(a.dist + v);
     uint32_t total = 0;
     const uint32_t excl = wave_excl_scan(deg, total);
-    st.v[kStatRelax] += total;
     uint32_t out_n = 0;
     bool ok = true;
+    bool keep_known = false;
+    bool keep = false;
     for (uint32_t base = 0; base < total && ok; base += kWave) {
         const uint32_t qi = base + lane;
+        const bool act = qi < total;
         uint32_t lo = 0;
 #pragma unroll
         for (uint32_t step = 32; step > 0; step >>= 1) {
@@ -88,48 +118,79 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
         const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)lo, kWave);
         const uint32_t o_start = (uint32_t)__shfl((int)start, (int)lo, kWave);
         const uint32_t o_d = (uint32_t)__shfl((int)d, (int)lo, kWave);
-        bool improved = false;
-        uint64_t nrec = 0;
-        if (qi < total) {
+        uint32_t dst = 0, wt = 0;
+        if (act) {
             const uint32_t e = o_start + (qi - o_excl);
-            const uint32_t dst = a.col[e];
-            const uint32_t nd = o_d + a.wgt[e];
-            const uint32_t old = ld_relaxed(a.dist + dst);
-            if (nd < old) {
-                const uint32_t prev = fetch_min(a.dist + dst, nd);
-                improved = nd < prev;
-                st.v[kStatAtomic] += 1;   // per-lane count, summed at exit
-            }
-            nrec = make_rec(dst, nd);
+            dst = a.col[e];
+            wt = a.wgt[e];
         }
+        if (!keep_known) {
+            keep = valid && d <= cur;
+            keep_known = true;
+            st.v[kStatStale] += popc(ballot(valid && !keep));
+        }
+        const bool o_keep = __shfl((int)keep, (int)lo, kWave) != 0;
+        const bool go = act && o_keep;
+        bool improved = false;
+        uint32_t nd = 0, cs = 0, ce = 0;
+        if (go) {
+            nd = o_d + wt;
+            const uint32_t prev = fetch_min(a.dist + dst, nd);
+            cs = a.row[dst];
+            ce = a.row[dst + 1];
+            improved = nd < prev;
+        }
+        const uint64_t gm = ballot(go);
         const uint64_t im = ballot(improved);
-        if (improved) out[out_n + popc(im & lanemask_lt())] = nrec;
+        st.v[kStatRelax] += popc(gm);
+        st.v[kStatAtomic] += popc(gm);
+        st.v[kStatImprove] += popc(im);
+        if (improved) {
+            const uint32_t p = out_n + popc(im & lanemask_lt());
+            out.rec[p] = make_rec(dst, nd);
+            out.row[p] = make_row(cs, ce - cs);
+        }
         wave_sync();
         out_n += popc(im);
-        st.v[kStatImprove] += popc(im);
-        if (out_n >= kWave) ok = flush_out<kSLF>(out, out_n, kWave, ring, l1, a.q, st);
+        clk.lap(st, kClkEdges);
+        if (out_n >= kWave) {
+            ok = flush_out<kSLF>(out, out_n, kWave, ring, l1, a.q, st);
+            clk.lap(st, kClkFlush);
+        }
     }
-    if (ok && out_n) ok = flush_out<kSLF>(out, out_n, out_n, ring, l1, a.q, st);
+    if (!keep_known) {
+        // batch without edges: still count stale records
+        st.v[kStatStale] += popc(ballot(valid && d > cur));
+    }
+    if (ok && out_n) {
+        ok = flush_out<kSLF>(out, out_n, out_n, ring, l1, a.q, st);
+        clk.lap(st, kClkFlush);
+    }
     return ok;
 }
 
-template <int kWaves, bool kSLF>
+template <int kWaves, bool kSLF, bool kProf>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
-    __shared__ uint64_t s_ring[kWaves][kL1Cap];
-    __shared__ uint64_t s_out[kWaves][kOutCap];
+    __shared__ uint64_t s_ring_rec[kWaves][kL1Cap];
+    __shared__ uint64_t s_ring_row[kWaves][kL1Cap];
+    __shared__ uint64_t s_out_rec[kWaves][kOutCap];
+    __shared__ uint64_t s_out_row[kWaves][kOutCap];
     __shared__ uint64_t s_stage[kWaves][kWave];
 
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / kWave;
     const uint32_t gwave = blockIdx.x * kWaves + w;
-    uint64_t *ring = s_ring[w];
-    uint64_t *out = s_out[w];
+    const L1Ring ring{s_ring_rec[w], s_ring_row[w]};
+    const L1Ring out{s_out_rec[w], s_out_row[w]};
     uint64_t *stage = s_stage[w];
     const L2Queue &q = a.q;
 
     WaveStats st;
 #pragma unroll
     for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    PhaseClock<kProf> clk;
+    const uint64_t t_begin = kProf ? clock64() : 0;
+    clk.start();
 
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
     L1State l1{0u, 0u};
@@ -139,24 +200,31 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     uint32_t sleep = 0;
 
     for (;;) {
-        uint64_t rec = kEmpty;
+        uint64_t rec = kEmpty, rowi = kNoRow;
         uint32_t nin = 0;
-        if (kSLF) nin = l1_pop(ring, l1, kWave, 0, rec);
+        if (kSLF) {
+            nin = l1_pop(ring, l1, rec, rowi);
+            clk.lap(st, kClkL1Pop);
+        }
         if (nin < a.min_read) {  // here L1 is empty
             const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, st);
             if (got) {
                 credit += got;
                 const uint32_t direct = got < kWave - nin ? got : kWave - nin;
-                if (lane >= nin && lane < nin + direct) rec = stage[lane - nin];
+                if (lane >= nin && lane < nin + direct) {
+                    rec = stage[lane - nin];
+                    rowi = kNoRow;
+                }
                 const uint32_t rest = got - direct;
                 if (rest) {
                     uint64_t r2 = kEmpty;
                     if (lane < rest) r2 = stage[direct + lane];
                     wave_sync();
-                    if (!l1_push(ring, l1, r2, rest, q, st)) break;
+                    if (!l1_push(ring, l1, r2, kNoRow, rest, q, st)) break;
                 }
                 nin += direct;
             }
+            clk.lap(st, kClkL2Pop);
         }
         if (nin == 0) {
             if (credit) {
@@ -172,14 +240,16 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
             }
             sleep = sleep ? (sleep * 2 < a.max_sleep ? sleep * 2 : a.max_sleep) : 1;
             for (uint32_t k = 0; k < sleep; ++k) __builtin_amdgcn_s_sleep(4);
+            clk.lap(st, kClkIdle);
             continue;
         }
         sleep = 0;
         st.v[kStatBatches] += 1;
-        if (!process_batch<kSLF>(a, rec, nin, out, ring, l1, st)) break;
+        if (!process_batch<kSLF, kProf>(a, rec, rowi, nin, out, ring, l1, st, clk)) break;
         if (credit && l1.size == 0) {
             if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
             credit = 0;
+            clk.lap(st, kClkCredit);
         }
         if ((st.v[kStatBatches] & 31) == 0) {
             if (uniform(ld_relaxed(q.ctl + kExitOff))) break;
@@ -189,9 +259,8 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
             }
         }
     }
+    if (kProf) st.v[kClkTotal] = clock64() - t_begin;
 
-    // Per-lane atomic counts are summed over the wave; the rest are uniform.
-    st.v[kStatAtomic] = wave_sum((uint32_t)st.v[kStatAtomic]);
     if (lane == 0) {
         uint64_t *stats = reinterpret_cast<uint64_t *>(q.ctl + kStatsOff);
 #pragma unroll
@@ -268,8 +337,8 @@ hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, h
 }
 
 static const void *solver_fn(bool slf) {
-    return slf ? reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true>)
-               : reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, false>);
+    return slf ? reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true, false>)
+               : reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, false, false>);
 }
 
 hipError_t solver_occupancy(bool slf, int *blocks_per_cu) {
@@ -295,12 +364,15 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     a.min_read = L.min_read;
     a.max_sleep = L.max_sleep;
     a.timeout_ticks = L.timeout_ticks;
-    if (L.slf)
-        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true>), dim3(L.blocks),
-                           dim3(kWavesPerBlock * kWave), 0, s, a);
+    const dim3 grid(L.blocks), block(kWavesPerBlock * kWave);
+    if (L.slf && !L.profile)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false>), grid, block, 0, s, a);
+    else if (L.slf)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true>), grid, block, 0, s, a);
+    else if (!L.profile)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, false>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false>), dim3(L.blocks),
-                           dim3(kWavesPerBlock * kWave), 0, s, a);
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, true>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 

