@@ -76,7 +76,8 @@ typedef struct mlmq_params {
     double   timeout_s;       /* device watchdog; <= 0 means 60 s */
     int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
     int32_t  profile;         /* 1: per-phase clock build of the kernel (mlmq_timing.phase_cycles) */
-    int32_t  reserved[6];
+    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 64 */
+    int32_t  reserved[5];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -101,7 +102,11 @@ typedef struct mlmq_timing {
      * L1 pop, L2 pop, CSR-range fetch, edge relaxation, push/flush, idle,
      * total, credit return */
     uint64_t phase_cycles[8];
-    uint64_t reserved[6];
+    /* params.profile = 1 only: cycles in owner computation, edge read (incl.
+     * the stale check wait), atomicMin round trip; 16 empty laps (clock
+     * overhead); edge chunks processed */
+    uint64_t detail_cycles[5];
+    uint64_t reserved[1];
 } mlmq_timing;
 
 /* Reference graph statistics struct (core/include/graph_info.h:8-20). */

@@ -50,7 +50,7 @@ class Params(ctypes.Structure):
         ("l2_batch", ctypes.c_int32), ("min_read", ctypes.c_int32),
         ("bucket_capacity", ctypes.c_int64), ("timeout_s", ctypes.c_double),
         ("auto_retry", ctypes.c_int32), ("profile", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("l1_pop", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -65,13 +65,16 @@ class Timing(ctypes.Structure):
         ("improvements", ctypes.c_uint64), ("stale", ctypes.c_uint64),
         ("batches", ctypes.c_uint64), ("idle_polls", ctypes.c_uint64),
         ("retries", ctypes.c_uint64), ("phase_cycles", ctypes.c_uint64 * 8),
-        ("reserved", ctypes.c_uint64 * 6),
+        ("detail_cycles", ctypes.c_uint64 * 5), ("reserved", ctypes.c_uint64 * 1),
     ]
+    DETAILS = ("owner", "edge_read", "atomic", "laps16", "chunks")
     PHASES = ("l1_pop", "l2_pop", "row_fetch", "edges", "flush", "idle", "total", "credit")
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "phase_cycles")}
+        d = {k: getattr(self, k) for k, _ in self._fields_
+             if k not in ("reserved", "phase_cycles", "detail_cycles")}
         d["phase_cycles"] = dict(zip(self.PHASES, list(self.phase_cycles)))
+        d["detail_cycles"] = dict(zip(self.DETAILS, list(self.detail_cycles)))
         return d
 
 

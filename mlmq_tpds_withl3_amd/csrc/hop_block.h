@@ -1,0 +1,52 @@
+// Hop blocks: a 64-byte, 64-byte-aligned per-vertex copy of the CSR row.
+//
+//   word 0      out-degree
+//   word 1      CSR start offset (row[v])
+//   words 2..15 up to kInlineEdges = 7 out-edges as (destination, weight)
+//
+// A vertex with at most 7 out-edges (99.5 % of the road-like inputs, whose
+// maximum degree is 9) is fully described by its block, so relaxing it needs
+// no row-offset or column load: one aligned 64-byte load replaces the
+// row -> col dependency chain of CSR.  Heavier vertices keep using the CSR
+// arrays through the start offset.  The blocks are built on the device from
+// the CSR when a graph is loaded; the CSR stays the graph of record.
+#pragma once
+
+#include "device_common.h"
+
+namespace mlmq {
+
+constexpr uint32_t kInlineEdges = 7;
+
+struct HopBlock {
+    uint4 q[4];
+    __device__ __forceinline__ uint32_t deg() const { return q[0].x; }
+    __device__ __forceinline__ uint32_t start() const { return q[0].y; }
+};
+
+__device__ __forceinline__ HopBlock load_block(const uint4 *blocks, uint32_t v) {
+    const uint4 *p = blocks + 4ull * v;
+    HopBlock b;
+    b.q[0] = p[0];
+    b.q[1] = p[1];
+    b.q[2] = p[2];
+    b.q[3] = p[3];
+    return b;
+}
+
+__device__ __forceinline__ HopBlock empty_block() {
+    HopBlock b;
+    b.q[0] = b.q[1] = b.q[2] = b.q[3] = make_uint4(0u, 0u, 0u, 0u);
+    return b;
+}
+
+// Edge k (< kInlineEdges) of the block stored as plane-major entry `o` of an
+// LDS block array with `cap` entries per plane.
+__device__ __forceinline__ uint2 block_edge_lds(const uint4 *planes, uint32_t cap, uint32_t o,
+                                                uint32_t k) {
+    const uint32_t word = 2 + 2 * k;
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(planes + (word >> 2) * cap + o);
+    return *reinterpret_cast<const uint2 *>(p + (word & 3));
+}
+
+}  // namespace mlmq

@@ -100,10 +100,11 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
     return true;
 }
 
-// Pop up to one claim's worth of ready records into `stage[0..got)`.
-// `cl_start`/`cl_mask` carry the wave's claims: lane b holds bucket b.
+// Pop up to one claim's worth (at most `room`) of ready records into
+// `stage[0..got)`.  `cl_start`/`cl_mask` carry the wave's claims: lane b
+// holds bucket b.
 __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
-                           uint64_t *stage, WaveStats &st) {
+                           uint64_t *stage, uint32_t room, WaveStats &st) {
     const uint32_t lane = lane_id();
     const bool mine = lane < q.nbuckets;
     // one round trip: the epoch and every bucket's counters
@@ -132,10 +133,13 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
             s = read_lane(s0, 0);
             mk = kk >= 64 ? ~0ull : ((1ull << kk) - 1);
         }
-        // Take every claimed slot that holds a record; an empty slot is left
-        // empty by the exchange (a writer's later CAS still succeeds).
+        // Take every claimed slot (up to `room`) that holds a record; an empty
+        // slot is left empty by the exchange (a writer's later CAS still
+        // succeeds).
+        uint64_t try_mask = mk;
+        while (popc(try_mask) > room) try_mask &= ~(1ull << (63 - __builtin_clzll(try_mask)));
         uint64_t v = kEmpty;
-        if ((mk >> lane) & 1) v = xchg64(slot_ptr(q, b, s + lane), kEmpty);
+        if ((try_mask >> lane) & 1) v = xchg64(slot_ptr(q, b, s + lane), kEmpty);
         const uint64_t taken = ballot(v != kEmpty);
         mk &= ~taken;
         if (lane == b) {

@@ -28,6 +28,7 @@ struct mlmq_ctx {
     int32_t n = 0;
     int64_t m = 0;
     uint32_t *row = nullptr, *col = nullptr, *wgt = nullptr;
+    uint4 *blocks = nullptr;   // hop blocks, always owned by the context
     bool owns_graph = false;
     mlmq_graph_info info{};
     // per-query state
@@ -72,6 +73,8 @@ void free_graph(mlmq_ctx *c) {
         (void)hipFree(c->col);
         (void)hipFree(c->wgt);
     }
+    if (c->blocks) (void)hipFree(c->blocks);
+    c->blocks = nullptr;
     c->row = c->col = c->wgt = nullptr;
     c->owns_graph = false;
     c->n = 0;
@@ -135,6 +138,14 @@ int validate_csr(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const in
     return MLMQ_OK;
 }
 
+// Device-side hop blocks (64 B per vertex) from the device CSR.
+int build_blocks(mlmq_ctx *c) {
+    HIP_TRY(c, hipMalloc(&c->blocks, 64 * (size_t)c->n));
+    HIP_TRY(c, mlmq::launch_build_blocks(c->row, c->col, c->wgt, (uint32_t)c->n, c->blocks, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MLMQ_OK;
+}
+
 int upload(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t *col,
            const int32_t *w) {
     int rc = validate_csr(c, n, m, row, col, w);
@@ -157,7 +168,7 @@ int upload(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t 
     c->n = n;
     c->m = m;
     compute_info(&c->info, n, m, row, nullptr, w);
-    return MLMQ_OK;
+    return build_blocks(c);
 }
 
 uint32_t ceil_log2(uint64_t x) {
@@ -219,6 +230,8 @@ void mlmq_params_default(mlmq_params *p) {
     p->bucket_capacity = 0;
     p->timeout_s = 60.0;
     p->auto_retry = 1;
+    p->profile = 0;
+    p->l1_pop = 64;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -347,7 +360,7 @@ int mlmq_attach_device_csr(mlmq_ctx *c, int32_t n, int64_t m, const int32_t *row
     c->n = n;
     c->m = m;
     compute_info(&c->info, n, m, nullptr, row.data(), w.data());
-    return MLMQ_OK;
+    return build_blocks(c);
 }
 
 int mlmq_graph_stats(mlmq_ctx *c, mlmq_graph_info *info) {
@@ -390,6 +403,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(c, MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
     if (p.l2_batch < 1 || p.l2_batch > 64) return fail(c, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
     if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
+    if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -428,6 +442,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.row = c->row;
         L.col = c->col;
         L.wgt = c->wgt;
+        L.blocks = c->blocks;
         L.dist = c->dist;
         L.slots = c->slots;
         L.ctl = c->ctl;
@@ -440,8 +455,9 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.spin_limit = 1u << 22;
         L.min_read = (uint32_t)p.min_read;
         L.max_sleep = 16;
+        L.pop_max = (uint32_t)p.l1_pop;
         L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
-        L.blocks = blocks;
+        L.grid_blocks = blocks;
         L.slf = slf;
         L.profile = p.profile != 0;
 
@@ -504,6 +520,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
             timing->idle_polls = st[mlmq::kStatIdle];
             timing->retries = retries;
             for (int i = 0; i < 8; ++i) timing->phase_cycles[i] = st[mlmq::kClkL1Pop + i];
+            for (int i = 0; i < 5; ++i) timing->detail_cycles[i] = st[mlmq::kClkOwner + i];
         }
         return MLMQ_OK;
     }

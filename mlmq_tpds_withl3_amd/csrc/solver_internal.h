@@ -13,6 +13,7 @@ struct SolverLaunch {
     const uint32_t *row;
     const uint32_t *col;
     const uint32_t *wgt;
+    const uint4 *blocks;
     uint32_t *dist;
     uint64_t *slots;
     uint32_t *ctl;
@@ -25,8 +26,9 @@ struct SolverLaunch {
     uint32_t spin_limit;
     uint32_t min_read;
     uint32_t max_sleep;
+    uint32_t pop_max;
     uint64_t timeout_ticks;
-    int blocks;
+    int grid_blocks;
     bool slf;
     bool profile;   // per-phase clock build of the kernel
 };
@@ -35,6 +37,8 @@ hipError_t launch_reset(uint32_t *dist, uint32_t n, uint32_t src, uint32_t *ctl,
                         hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t count, uint64_t value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);
+hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
+                               uint32_t n, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);
 hipError_t solver_occupancy(bool slf, int *blocks_per_cu);
 hipError_t launch_solver(const SolverLaunch &L, hipStream_t s);

@@ -21,17 +21,17 @@
 
 namespace mlmq {
 
-constexpr uint32_t kOutCap = 128;   // pending pushes (reference node_out: 2 x node_size)
-
 struct SolveArgs {
-    const uint32_t *row;   // n + 1 CSR offsets
-    const uint32_t *col;   // m destinations
-    const uint32_t *wgt;   // m weights
-    uint32_t *dist;        // n distances
+    const uint32_t *row;    // n + 1 CSR offsets
+    const uint32_t *col;    // m destinations
+    const uint32_t *wgt;    // m weights
+    const uint4 *blocks;    // n hop blocks (4 x uint4 each)
+    uint32_t *dist;         // n distances
     L2Queue q;
     uint32_t n;
     uint32_t min_read;
-    uint32_t max_sleep;    // idle back-off cap (x 256 cycles)
+    uint32_t max_sleep;     // idle back-off cap (x 256 cycles)
+    uint32_t pop_max;       // records taken from L1 per batch (<= 64)
     uint64_t timeout_ticks;
 };
 
@@ -50,138 +50,120 @@ struct PhaseClock {
     }
 };
 
-// Flush the newest `cnt` (<= 64) pending entries of `out` to L1 (or L2).
-template <bool kSLF>
-__device__ __forceinline__ bool flush_out(const L1Ring &out, uint32_t &out_n, uint32_t cnt,
-                                          const L1Ring &ring, L1State &l1, const L2Queue &q,
-                                          WaveStats &st) {
-    const uint32_t lane = lane_id();
-    uint64_t r = kEmpty, w = kNoRow;
-    if (lane < cnt) {
-        r = out.rec[out_n - cnt + lane];
-        w = out.row[out_n - cnt + lane];
-    }
-    wave_sync();
-    out_n -= cnt;
-    if (kSLF) return l1_push(ring, l1, r, w, cnt, q, st);
-    return l2_push(q, r, lane < cnt, st);
-}
-
 // Relax all out-edges of the records in lanes [0, nin) (simple_process,
-// SS/SSSP/sssp_run.cu:143-413).  A record is skipped when its distance is
-// above the vertex's current one (reference filter :174, process on <=).
-// Edge slots are spread over the 64 lanes by an exclusive scan of degrees;
-// each lane finds its owner by a binary search over the scan held in lanes.
-// A lane pushes (dst, nd) only if its atomicMin strictly lowered dist[dst]
-// (:211-243).
+// SS/SSSP/sssp_run.cu:143-413).  `in` holds the records' hop blocks.
 //
-// Latency shape (one hop = two dependent round trips): the stale check of
-// the batch is in flight together with the first chunk's col/weight loads,
-// and each relaxation's atomicMin is in flight together with the loads of
-// dst's CSR range, which travels with the pushed record.
+//  * Stale filter (reference :174, process on <=): dist[v] is loaded at the
+//    start of the batch and only waited for just before the first atomicMin,
+//    so it overlaps the scan and the edge reads.
+//  * Load balance: edge slots are spread over the 64 lanes by a DPP scan of
+//    degrees; the owner of a slot is a DPP running maximum of start markers
+//    (lane + 1 written at each record's first slot), carried across chunks.
+//  * Edges of inline vertices come from LDS; heavier ones from the CSR.
+//  * A lane pushes (dst, nd) only if its atomicMin strictly lowered dst's
+//    distance (:211-243); dst's hop block is loaded in the same round trip as
+//    the atomic and travels with the pushed record, so the next hop starts
+//    without any graph load.
 template <bool kSLF, bool kProf>
-__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint64_t rowi, uint32_t nin,
-                              const L1Ring &out, const L1Ring &ring, L1State &l1, WaveStats &st,
+__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t bdeg, uint32_t bstart,
+                              uint32_t nin, const BlockRing &in, uint32_t *map,
+                              const BlockRing &ring, L1State &l1, WaveStats &st,
                               PhaseClock<kProf> &clk) {
     const uint32_t lane = lane_id();
     const bool valid = lane < nin;
     const uint32_t v = rec_vertex(rec);
     const uint32_t d = rec_dist(rec);
-    uint32_t start = (uint32_t)rowi, deg = (uint32_t)(rowi >> 32);
-    // Records that came through L2 carry no CSR range.
-    const bool need_row = valid && rowi == kNoRow;
-    if (ballot(need_row)) {
-        if (need_row) {
-            start = a.row[v];
-            deg = a.row[v + 1] - start;
-        }
-        clk.lap(st, kClkRow);
-    }
-    if (!valid) deg = 0;
+    const uint32_t deg = valid ? bdeg : 0u;
+    const bool inl = deg <= kInlineEdges;
+    const uint32_t start_inl = bstart | (inl ? 0x80000000u : 0u);
     uint32_t cur = 0;
     if (valid) cur = ld_relaxed(a.dist + v);
     uint32_t total = 0;
     const uint32_t excl = wave_excl_scan(deg, total);
-    uint32_t out_n = 0;
+    bool keep_known = false, keep = false;
+    uint32_t carry = 0;
     bool ok = true;
-    bool keep_known = false;
-    bool keep = false;
     for (uint32_t base = 0; base < total && ok; base += kWave) {
         const uint32_t qi = base + lane;
         const bool act = qi < total;
-        uint32_t lo = 0;
-#pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1) {
-            const uint32_t cand = lo + step;
-            if ((uint32_t)__shfl((int)excl, (int)cand, kWave) <= qi) lo = cand;
-        }
-        const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)lo, kWave);
-        const uint32_t o_start = (uint32_t)__shfl((int)start, (int)lo, kWave);
-        const uint32_t o_d = (uint32_t)__shfl((int)d, (int)lo, kWave);
-        uint32_t dst = 0, wt = 0;
+        map[lane] = 0;
+        wave_sync();
+        if (deg > 0 && excl >= base && excl - base < (uint32_t)kWave) map[excl - base] = lane + 1;
+        wave_sync();
+        const uint32_t o1 = max(carry, wave_incl_max(map[lane]));
+        carry = read_lane(o1, kWave - 1);
+        const uint32_t o = act ? o1 - 1 : 0u;
+        const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)o, kWave);
+        const uint32_t o_si = (uint32_t)__shfl((int)start_inl, (int)o, kWave);
+        const uint32_t k = qi - o_excl;
+        st.v[kStatChunks] += 1;
+        clk.lap(st, kClkOwner);
+        uint2 cw = make_uint2(0u, 0u);
         if (act) {
-            const uint32_t e = o_start + (qi - o_excl);
-            dst = a.col[e];
-            wt = a.wgt[e];
+            if (o_si >> 31) {
+                cw = block_edge_lds(in.blk, in.cap, o, k);
+            } else {
+                const uint32_t e = (o_si & 0x7FFFFFFFu) + k;
+                cw = make_uint2(a.col[e], a.wgt[e]);
+            }
         }
         if (!keep_known) {
             keep = valid && d <= cur;
             keep_known = true;
             st.v[kStatStale] += popc(ballot(valid && !keep));
         }
-        const bool o_keep = __shfl((int)keep, (int)lo, kWave) != 0;
-        const bool go = act && o_keep;
+        const uint32_t o_dk = (uint32_t)__shfl((int)(d | (keep ? 0x80000000u : 0u)), (int)o, kWave);
+        const bool go = act && (o_dk >> 31);
+        if (kProf) {
+            // make the lap wait for the edge reads and the stale check
+            if (ballot(go && cw.x == 0xFFFFFFFFu)) st.v[kStatSpare0] += 1;
+            clk.lap(st, kClkEdgeRead);
+        }
         bool improved = false;
-        uint32_t nd = 0, cs = 0, ce = 0;
+        uint32_t nd = 0;
+        HopBlock cb = empty_block();
         if (go) {
-            nd = o_d + wt;
-            const uint32_t prev = fetch_min(a.dist + dst, nd);
-            cs = a.row[dst];
-            ce = a.row[dst + 1];
+            nd = (o_dk & 0x7FFFFFFFu) + cw.y;
+            const uint32_t prev = fetch_min(a.dist + cw.x, nd);
+            cb = load_block(a.blocks, cw.x);
             improved = nd < prev;
+        }
+        if (kProf) {
+            if (ballot(go && (cb.q[3].w == 0xFFFFFFFFu))) st.v[kStatSpare1] += 1;
+            clk.lap(st, kClkAtomic);
         }
         const uint64_t gm = ballot(go);
         const uint64_t im = ballot(improved);
         st.v[kStatRelax] += popc(gm);
         st.v[kStatAtomic] += popc(gm);
         st.v[kStatImprove] += popc(im);
-        if (improved) {
-            const uint32_t p = out_n + popc(im & lanemask_lt());
-            out.rec[p] = make_rec(dst, nd);
-            out.row[p] = make_row(cs, ce - cs);
-        }
-        wave_sync();
-        out_n += popc(im);
         clk.lap(st, kClkEdges);
-        if (out_n >= kWave) {
-            ok = flush_out<kSLF>(out, out_n, kWave, ring, l1, a.q, st);
+        if (im) {
+            const uint64_t nrec = make_rec(cw.x, nd);
+            if (kSLF) ok = l1_push(ring, l1, nrec, cb, im, a.q, st);
+            else ok = l2_push(a.q, nrec, improved, st);
             clk.lap(st, kClkFlush);
         }
     }
-    if (!keep_known) {
-        // batch without edges: still count stale records
-        st.v[kStatStale] += popc(ballot(valid && d > cur));
-    }
-    if (ok && out_n) {
-        ok = flush_out<kSLF>(out, out_n, out_n, ring, l1, a.q, st);
-        clk.lap(st, kClkFlush);
-    }
+    if (!keep_known) st.v[kStatStale] += popc(ballot(valid && d > cur));
     return ok;
 }
 
 template <int kWaves, bool kSLF, bool kProf>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
     __shared__ uint64_t s_ring_rec[kWaves][kL1Cap];
-    __shared__ uint64_t s_ring_row[kWaves][kL1Cap];
-    __shared__ uint64_t s_out_rec[kWaves][kOutCap];
-    __shared__ uint64_t s_out_row[kWaves][kOutCap];
+    __shared__ uint4 s_ring_blk[kWaves][4 * kL1Cap];
+    __shared__ uint64_t s_in_rec[kWaves][kWave];
+    __shared__ uint4 s_in_blk[kWaves][4 * kWave];
+    __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
 
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / kWave;
     const uint32_t gwave = blockIdx.x * kWaves + w;
-    const L1Ring ring{s_ring_rec[w], s_ring_row[w]};
-    const L1Ring out{s_out_rec[w], s_out_row[w]};
+    const BlockRing ring{s_ring_rec[w], s_ring_blk[w], kL1Cap};
+    const BlockRing in{s_in_rec[w], s_in_blk[w], (uint32_t)kWave};
+    uint32_t *map = s_map[w];
     uint64_t *stage = s_stage[w];
     const L2Queue &q = a.q;
 
@@ -191,6 +173,9 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     PhaseClock<kProf> clk;
     const uint64_t t_begin = kProf ? clock64() : 0;
     clk.start();
+    if (kProf) {
+        for (int i = 0; i < 16; ++i) clk.lap(st, kClkLaps);
+    }
 
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
     L1State l1{0u, 0u};
@@ -200,29 +185,23 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     uint32_t sleep = 0;
 
     for (;;) {
-        uint64_t rec = kEmpty, rowi = kNoRow;
+        uint64_t rec = kEmpty;
+        HopBlock b = empty_block();
         uint32_t nin = 0;
         if (kSLF) {
-            nin = l1_pop(ring, l1, rec, rowi);
+            nin = l1_pop(ring, l1, a.pop_max, rec, b);
             clk.lap(st, kClkL1Pop);
         }
         if (nin < a.min_read) {  // here L1 is empty
-            const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, st);
+            const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, kWave - nin, st);
             if (got) {
                 credit += got;
-                const uint32_t direct = got < kWave - nin ? got : kWave - nin;
-                if (lane >= nin && lane < nin + direct) {
+                if (lane >= nin && lane < nin + got) {
                     rec = stage[lane - nin];
-                    rowi = kNoRow;
+                    b = load_block(a.blocks, rec_vertex(rec));
                 }
-                const uint32_t rest = got - direct;
-                if (rest) {
-                    uint64_t r2 = kEmpty;
-                    if (lane < rest) r2 = stage[direct + lane];
-                    wave_sync();
-                    if (!l1_push(ring, l1, r2, kNoRow, rest, q, st)) break;
-                }
-                nin += direct;
+                nin += got;
+                clk.lap(st, kClkRow);
             }
             clk.lap(st, kClkL2Pop);
         }
@@ -245,7 +224,11 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
         }
         sleep = 0;
         st.v[kStatBatches] += 1;
-        if (!process_batch<kSLF, kProf>(a, rec, rowi, nin, out, ring, l1, st, clk)) break;
+        if (lane < nin) ring_put(in, lane, rec, b);
+        wave_sync();
+        if (!process_batch<kSLF, kProf>(a, rec, b.deg(), b.start(), nin, in, map, ring, l1, st,
+                                        clk))
+            break;
         if (credit && l1.size == 0) {
             if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
             credit = 0;
@@ -266,6 +249,31 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
 #pragma unroll
         for (uint32_t i = 0; i < kNumStats; ++i)
             if (st.v[i]) add64(stats + i, st.v[i]);
+    }
+}
+
+// Hop blocks from the CSR (layout in hop_block.h).
+__global__ void build_blocks_kernel(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
+                                    uint32_t n, uint4 *blocks) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t s = row[v], deg = row[v + 1] - s;
+        uint32_t wd[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) wd[i] = 0;
+        wd[0] = deg;
+        wd[1] = s;
+        if (deg <= kInlineEdges) {
+            for (uint32_t k = 0; k < deg; ++k) {
+                wd[2 + 2 * k] = col[s + k];
+                wd[3 + 2 * k] = wgt[s + k];
+            }
+        }
+        uint4 *p = blocks + 4ull * v;
+        p[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        p[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+        p[2] = make_uint4(wd[8], wd[9], wd[10], wd[11]);
+        p[3] = make_uint4(wd[12], wd[13], wd[14], wd[15]);
     }
 }
 
@@ -330,6 +338,13 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStrea
     return hipGetLastError();
 }
 
+hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
+                               uint32_t n, uint4 *blocks, hipStream_t s) {
+    hipLaunchKernelGGL(build_blocks_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, row, col, wgt,
+                       n, blocks);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s) {
     hipLaunchKernelGGL(copy_u32_kernel, dim3(grid_for(count, 256)), dim3(256), 0, s, dst, src,
                        count);
@@ -351,6 +366,7 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     a.row = L.row;
     a.col = L.col;
     a.wgt = L.wgt;
+    a.blocks = L.blocks;
     a.dist = L.dist;
     a.q.slots = L.slots;
     a.q.ctl = L.ctl;
@@ -363,8 +379,9 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     a.n = L.n;
     a.min_read = L.min_read;
     a.max_sleep = L.max_sleep;
+    a.pop_max = L.pop_max;
     a.timeout_ticks = L.timeout_ticks;
-    const dim3 grid(L.blocks), block(kWavesPerBlock * kWave);
+    const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
     if (L.slf && !L.profile)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false>), grid, block, 0, s, a);
     else if (L.slf)
