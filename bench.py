@@ -49,10 +49,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="road6m")
-    ap.add_argument("--delta", type=int, default=None)
+    ap.add_argument("--delta", type=int, default=None,
+                    help="bucket width; default: best of the workload's delta sweep")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a plain copy kernel of this many bytes (counter calibration)")
     ap.add_argument("--queue", default="L1SLF_L2DQ")
     ap.add_argument("--work-blocks", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--l1-pop", type=int, default=32)
     ap.add_argument("--l2-batch", type=int, default=8)
     ap.add_argument("--min-read", type=int, default=8)
     ap.add_argument("--buckets", type=int, default=16)
@@ -72,19 +76,37 @@ def main():
     import numpy as np
     from mlmq_tpds_withl3_amd import Solver, make_workload, default_params
 
+    if args.calibrate_copy:
+        with Solver(0) as s:
+            ms = [s.probe_copy(args.calibrate_copy) for _ in range(3)]
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "read_bytes": args.calibrate_copy,
+                          "write_bytes": args.calibrate_copy, "device_ms": ms,
+                          "gbs": 2 * args.calibrate_copy / (min(ms) / 1e3) / 1e9}), flush=True)
+        return
+
     t0 = time.time()
     g, src, spec = make_workload(args.workload)
     row, col, w = g.arrays()
     log(f"workload {args.workload}: n={g.n} m={g.m} src={src} gen {time.time() - t0:.1f}s")
-    delta = args.delta if args.delta is not None else spec["delta"]
-    params = default_params(queue=args.queue, delta=delta, work_blocks=args.work_blocks,
-                            blocks_per_cu=args.blocks_per_cu, l2_batch=args.l2_batch,
-                            min_read=args.min_read, buckets=args.buckets,
-                            bucket_window=args.window, timeout_s=30.0)
+    def make_params(delta):
+        return default_params(queue=args.queue, delta=delta, work_blocks=args.work_blocks,
+                              blocks_per_cu=args.blocks_per_cu, l2_batch=args.l2_batch,
+                              min_read=args.min_read, buckets=args.buckets,
+                              bucket_window=args.window, l1_pop=args.l1_pop, timeout_s=30.0)
 
     solver = Solver(0)
     solver.load_csr(row, col, w)
     shape = solver.device_shape()
+    sweep = {}
+    if args.delta is not None:
+        delta = args.delta
+    else:
+        # the workload's delta sweep, untimed: two solves per delta, keep the best
+        for dl in spec["delta_sweep"]:
+            sweep[dl] = min(solver.solve(src, make_params(dl))["device_ms"] for _ in range(2))
+        delta = min(sweep, key=sweep.get)
+        log(f"delta sweep (device ms): {sweep} -> {delta}")
+    params = make_params(delta)
     for i in range(args.warmup):
         t = solver.solve(src, params)
         log(f"warmup {i}: solve_ms={t['solve_ms']:.3f} device_ms={t['device_ms']:.3f}")
@@ -149,8 +171,9 @@ def main():
             "weight_scale": spec.get("weight_scale_applied"),
             "seed": spec.get("seed"),
             "queue": args.queue, "delta": delta,
+            "delta_sweep_device_ms": {str(k): round(v, 4) for k, v in sweep.items()},
             "buckets": args.buckets, "bucket_window": args.window,
-            "l2_batch": args.l2_batch, "min_read": args.min_read,
+            "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop,
             "work_blocks": last["work_blocks"], "waves": last["waves"],
             "cus": shape["cus"], "blocks_per_cu_max": shape["blocks_per_cu_max"],
         },

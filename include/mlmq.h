@@ -66,8 +66,9 @@ typedef struct mlmq_ctx mlmq_ctx;
 typedef struct mlmq_params {
     int32_t  queue_mode;      /* MLMQ_QUEUE_L1SLF_L2DQ (default) or MLMQ_QUEUE_L1N_L2DQ */
     uint32_t delta;           /* bucket width; reference BENCH_DELTA, default 200000 */
-    int32_t  work_blocks;     /* persistent workgroups; 0 = one per CU x blocks_per_cu */
-    int32_t  blocks_per_cu;   /* used when work_blocks == 0; 0 = default (2) */
+    int32_t  work_blocks;     /* persistent workgroups; 0 = automatic (blocks_per_cu x CUs if
+                                 blocks_per_cu > 0, else n / 25000 clamped to [32, 256]) */
+    int32_t  blocks_per_cu;   /* used when work_blocks == 0 */
     int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
     int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */
     int32_t  l2_batch;        /* records per L2 claim (l2_batch_size), 1..64, default 8 */
@@ -76,8 +77,10 @@ typedef struct mlmq_params {
     double   timeout_s;       /* device watchdog; <= 0 means 60 s */
     int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
     int32_t  profile;         /* 1: per-phase clock build of the kernel (mlmq_timing.phase_cycles) */
-    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 64 */
-    int32_t  reserved[5];
+    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 32 */
+    int32_t  l1_spill;        /* L1 holding more records than this after a push spills to L2;
+                                 1..64, default 64 */
+    int32_t  reserved[4];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -148,6 +151,12 @@ int mlmq_dist_device_ptr(mlmq_ctx *ctx, const int32_t **out);
 /* Occupancy query: resident workgroups per CU for the solver kernel and CU count. */
 int mlmq_device_shape(mlmq_ctx *ctx, int32_t *blocks_per_cu_max, int32_t *cus,
                       int32_t *threads_per_block);
+
+/* Calibration probe, no reference counterpart: one plain device-to-device
+ * copy kernel of `nbytes` (a multiple of 4), timed with HIP events.  Used to
+ * scale hardware byte counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) and as an
+ * HBM bandwidth reference. */
+int mlmq_probe_copy(mlmq_ctx *ctx, uint64_t nbytes, double *device_ms);
 
 /* ---- Reference driver ABI (SS/SSSP/main.cu:15-17), one implicit context ----
  * sssp_init takes a device CSR in the reference layout (int32, row[0] = 0).

@@ -26,7 +26,7 @@ SOLVER_SYMBOLS = [
     "mlmq_params_default", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
     "mlmq_load_csr", "mlmq_load_gr", "mlmq_attach_device_csr", "mlmq_graph_stats",
     "mlmq_graph_size", "mlmq_solve", "mlmq_copy_dist", "mlmq_dist_device_ptr",
-    "mlmq_device_shape", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
+    "mlmq_device_shape", "mlmq_probe_copy", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
 ]
 GRAPH_SYMBOLS = [
     "mlmq_graph_generate_knn", "mlmq_graph_generate_rmat", "mlmq_graph_read_gr",
@@ -50,7 +50,8 @@ class Params(ctypes.Structure):
         ("l2_batch", ctypes.c_int32), ("min_read", ctypes.c_int32),
         ("bucket_capacity", ctypes.c_int64), ("timeout_s", ctypes.c_double),
         ("auto_retry", ctypes.c_int32), ("profile", ctypes.c_int32),
-        ("l1_pop", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5),
+        ("l1_pop", ctypes.c_int32), ("l1_spill", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -118,6 +119,7 @@ def load() -> ctypes.CDLL:
         "mlmq_dist_device_ptr": (ctypes.c_int, [P, ctypes.POINTER(P)]),
         "mlmq_device_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                              ctypes.POINTER(i32)]),
+        "mlmq_probe_copy": (ctypes.c_int, [P, u64, ctypes.POINTER(ctypes.c_double)]),
         "sssp_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P, P, P]),
         "sssp_run_adaptive": (ctypes.c_int, [ctypes.c_int, GraphInfo]),
         "sssp_cp_data": (ctypes.c_int, [ctypes.POINTER(ctypes.POINTER(ctypes.c_int))]),

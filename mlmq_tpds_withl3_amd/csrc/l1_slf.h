@@ -1,116 +1,98 @@
 // L1: the per-wavefront "small label first" deque in LDS.
 //
 // Behaviour follows the reference's l1_SLF_queue (core/include/l1_queue.cuh:81-95,
-// core/src/l1_queue.cu:217-360), re-derived for 64-lane waves: a ring of
-// kL1Cap = 128 records (reference 64 for 32 lanes); a pop takes up to 64 from
-// the head (reference 32 = capacity / 2); a pushed record whose distance is
-// below the current head's (or, on an empty deque, below the first pushed
-// record's) goes to the front, every other record to the back; when the deque
-// holds more than kL1Spill = 64 records after a push, all of them move to L2.
+// core/src/l1_queue.cu:217-360): a ring of records; a pop takes records from
+// the head; a pushed record whose distance is below the current head's (or,
+// on an empty deque, below the first pushed record's) goes to the front,
+// every other record to the back; when the deque holds more than the spill
+// threshold after a push, all of its records move to L2.
 //
-// Differences: head index and size are wave-uniform values in scalar
-// registers; and each entry carries, beside the 8-byte (vertex, dist) record,
-// the vertex's 64-byte hop block (its out-edges, see hop_block.h), fetched in
-// the same round trip as the atomicMin that created the record.  Popping an
-// entry therefore needs no graph load at all.  Records that go through L2
-// travel without the block and fetch it again when popped.
+// Sizes are re-derived for 64-lane waves and many-edge pushes: the ring holds
+// kL1Cap = 512 8-byte records (4 KiB per wave); a pop takes at most 64 (one
+// per lane); the spill threshold is a launch parameter (default 64, the
+// reference's "half of the ring" rule at wave64 granularity).  One push
+// places all records a batch produced -- up to 64 lanes x 7 edges -- in a
+// single pass: positions come from ballots and popcounts only, so the writes
+// are issued back to back.  Head index and size are wave-uniform values in
+// scalar registers, not LDS words updated by lane 0 as in the reference.
 #pragma once
 
-#include "hop_block.h"
 #include "l2_delta_queue.h"
 
 namespace mlmq {
 
-constexpr uint32_t kL1Cap = 128;
+constexpr uint32_t kL1Cap = 512;
 constexpr uint32_t kL1Mask = kL1Cap - 1;
-constexpr uint32_t kL1Spill = 64;
-
-// A ring of records and their blocks, blocks stored as 4 planes of 16 bytes
-// (plane j of entry p at blk[j * cap + p]) so lanes reading consecutive
-// entries hit consecutive LDS addresses.
-struct BlockRing {
-    uint64_t *rec;
-    uint4 *blk;
-    uint32_t cap;
-};
 
 struct L1State {
     uint32_t head;
     uint32_t size;
 };
 
-__device__ __forceinline__ void ring_put(const BlockRing &r, uint32_t p, uint64_t rec,
-                                         const HopBlock &b) {
-    r.rec[p] = rec;
-    r.blk[0 * r.cap + p] = b.q[0];
-    r.blk[1 * r.cap + p] = b.q[1];
-    r.blk[2 * r.cap + p] = b.q[2];
-    r.blk[3 * r.cap + p] = b.q[3];
-}
-
-__device__ __forceinline__ void ring_get(const BlockRing &r, uint32_t p, uint64_t &rec,
-                                         HopBlock &b) {
-    rec = r.rec[p];
-    b.q[0] = r.blk[0 * r.cap + p];
-    b.q[1] = r.blk[1 * r.cap + p];
-    b.q[2] = r.blk[2 * r.cap + p];
-    b.q[3] = r.blk[3 * r.cap + p];
-}
-
-// Move up to `maxn` (<= 64) entries from the front into lanes [0, k) of
-// rec/blk (registers).
-__device__ __forceinline__ uint32_t l1_pop(const BlockRing &r, L1State &l1, uint32_t maxn,
-                                           uint64_t &rec, HopBlock &b) {
+// Move up to `maxn` (<= 64) records from the front into lanes [0, k).
+__device__ __forceinline__ uint32_t l1_pop(const uint64_t *ring, L1State &l1, uint32_t maxn,
+                                           uint64_t &rec) {
     const uint32_t k = l1.size < maxn ? l1.size : maxn;
-    if (lane_id() < k) ring_get(r, (l1.head + lane_id()) & kL1Mask, rec, b);
+    if (lane_id() < k) rec = ring[(l1.head + lane_id()) & kL1Mask];
     wave_sync();
     l1.head = (l1.head + k) & kL1Mask;
     l1.size -= k;
     return k;
 }
 
-// Insert the entries of the lanes in `mask` (at most 64; size <= 64 on entry).
-__device__ __forceinline__ void l1_insert(const BlockRing &r, L1State &l1, uint64_t rec,
-                                          const HopBlock &b, uint64_t mask) {
-    const uint32_t lane = lane_id();
-    const bool valid = (mask >> lane) & 1;
-    const uint32_t d = rec_dist(rec);
-    uint32_t pivot;
-    if (l1.size) pivot = uniform(rec_dist(r.rec[l1.head]));
-    else pivot = read_lane(d, (uint32_t)__builtin_ctzll(mask));
-    const bool front = valid && d < pivot;
-    const bool back = valid && !front;
-    const uint64_t fm = ballot(front);
-    const uint64_t bm = ballot(back);
-    const uint64_t lt = lanemask_lt();
-    uint32_t p = 0;
-    if (front) p = (l1.head - 1 - popc(fm & lt)) & kL1Mask;
-    if (back) p = (l1.head + l1.size + popc(bm & lt)) & kL1Mask;
-    if (valid) ring_put(r, p, rec, b);
-    wave_sync();
-    l1.head = (l1.head - popc(fm)) & kL1Mask;
-    l1.size += popc(mask);
-}
-
-// Push with the reference's write-back rule: above kL1Spill entries, the
-// whole deque goes to L2.  Returns false if L2 overflowed.
-__device__ bool l1_push(const BlockRing &r, L1State &l1, uint64_t rec, const HopBlock &b,
-                        uint64_t mask, const L2Queue &q, WaveStats &st) {
-    if (mask == 0) return true;
-    l1_insert(r, l1, rec, b, mask);
+// Spill every record of the deque to L2 (reference write-back of the whole
+// L1, core/src/l1_queue.cu:332-350).  Returns false if L2 overflowed.
+__device__ bool l1_spill_all(const uint64_t *ring, L1State &l1, const L2Queue &q, WaveStats &st) {
     bool ok = true;
-    if (l1.size > kL1Spill) {
-        while (l1.size && ok) {
-            const uint32_t k = l1.size < (uint32_t)kWave ? l1.size : (uint32_t)kWave;
-            uint64_t x = kEmpty;
-            if (lane_id() < k) x = r.rec[(l1.head + lane_id()) & kL1Mask];
-            wave_sync();
-            l1.head = (l1.head + k) & kL1Mask;
-            l1.size -= k;
-            ok = l2_push(q, x, lane_id() < k, st);
-        }
+    while (l1.size && ok) {
+        uint64_t x = kEmpty;
+        const uint32_t k = l1_pop(ring, l1, kWave, x);
+        ok = l2_push(q, x, lane_id() < k, st);
     }
     return ok;
+}
+
+// Push K registers of records: record k of lane i is present when bit i of
+// mask[k] is set; order is (k, lane).  Requires size <= spill on entry and
+// spill + K * 64 <= kL1Cap.
+template <int K>
+__device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
+                        const uint64_t (&mask)[K], uint32_t spill, const L2Queue &q,
+                        WaveStats &st) {
+    uint32_t total = 0;
+    uint32_t pivot = 0;
+    bool have_pivot = l1.size != 0;
+    if (have_pivot) pivot = uniform(rec_dist(ring[l1.head]));
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!have_pivot && mask[k]) {
+            pivot = read_lane(rec_dist(rec[k]), (uint32_t)__builtin_ctzll(mask[k]));
+            have_pivot = true;
+        }
+        total += popc(mask[k]);
+    }
+    if (total == 0) return true;
+    const uint32_t lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    uint32_t nf = 0, nb = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool valid = (mask[k] >> lane) & 1;
+        const bool front = valid && rec_dist(rec[k]) < pivot;
+        const uint64_t fm = ballot(front);
+        const uint64_t bm = mask[k] & ~fm;
+        uint32_t p = 0;
+        if (front) p = l1.head - 1 - (nf + popc(fm & lt));
+        else if (valid) p = l1.head + l1.size + nb + popc(bm & lt);
+        if (valid) ring[p & kL1Mask] = rec[k];
+        nf += popc(fm);
+        nb += popc(bm);
+    }
+    wave_sync();
+    l1.head = (l1.head - nf) & kL1Mask;
+    l1.size += total;
+    if (l1.size > spill) return l1_spill_all(ring, l1, q, st);
+    return true;
 }
 
 }  // namespace mlmq

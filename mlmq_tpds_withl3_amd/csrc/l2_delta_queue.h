@@ -107,14 +107,15 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
                            uint64_t *stage, uint32_t room, WaveStats &st) {
     const uint32_t lane = lane_id();
     const bool mine = lane < q.nbuckets;
-    // one round trip: the epoch and every bucket's counters
-    const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
+    // one round trip: every bucket's counters and the epoch (issued last so
+    // that waiting for it does not hold back the others)
     uint32_t t = 0, h = 0, k = 0;
     if (mine) {
         t = ld_relaxed(q.ctl + kTailOff + lane * kStride);
         h = ld_relaxed(q.ctl + kHeadOff + lane * kStride);
         k = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
     }
+    const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
     const bool unclaimed = mine && (int32_t)(t - h) > 0;
     const uint64_t avail_mask = ballot(unclaimed);
     const uint64_t claim_mask = ballot(mine && cl_mask != 0);

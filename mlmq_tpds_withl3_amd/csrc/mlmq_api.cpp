@@ -231,7 +231,8 @@ void mlmq_params_default(mlmq_params *p) {
     p->timeout_s = 60.0;
     p->auto_retry = 1;
     p->profile = 0;
-    p->l1_pop = 64;
+    p->l1_pop = 32;
+    p->l1_spill = 64;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -404,6 +405,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.l2_batch < 1 || p.l2_batch > 64) return fail(c, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
     if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
     if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
+    if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -412,10 +414,17 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     HIP_TRY(c, mlmq::solver_occupancy(slf, &occ));
     if (occ < 1) return fail(c, MLMQ_E_HIP, "solver kernel cannot be resident");
     int blocks = p.work_blocks;
-    if (blocks == 0) {
-        int per_cu = p.blocks_per_cu ? p.blocks_per_cu : 2;
-        if (per_cu > occ) per_cu = occ;
+    if (blocks == 0 && p.blocks_per_cu > 0) {
+        const int per_cu = p.blocks_per_cu < occ ? p.blocks_per_cu : occ;
         blocks = per_cu * c->cus;
+    } else if (blocks == 0) {
+        // Latency-bound solver: one wave per ~100 frontier vertices is enough.
+        // A road-like graph of n vertices keeps O(sqrt(n)) to O(n / 1000)
+        // records in flight; more waves only add queue polling.  Measured
+        // optimum: ~32 workgroups at 0.26 M vertices, ~256 at 6.3 M.
+        blocks = c->n / 25000;
+        if (blocks < 32) blocks = 32;
+        if (blocks > 256) blocks = 256;
     }
     if (blocks > occ * c->cus) blocks = occ * c->cus;
 
@@ -456,6 +465,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.min_read = (uint32_t)p.min_read;
         L.max_sleep = 16;
         L.pop_max = (uint32_t)p.l1_pop;
+        L.spill = (uint32_t)p.l1_spill;
         L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
         L.grid_blocks = blocks;
         L.slf = slf;
@@ -538,6 +548,32 @@ int mlmq_dist_device_ptr(mlmq_ctx *c, const int32_t **out) {
     if (!c || !out) return MLMQ_E_INVALID;
     if (!c->solved) return fail(c, MLMQ_E_NOGRAPH, "no completed solve");
     *out = (const int32_t *)c->dist;
+    return MLMQ_OK;
+}
+
+int mlmq_probe_copy(mlmq_ctx *c, uint64_t nbytes, double *device_ms) {
+    if (!c || nbytes < 4 || (nbytes & 3)) return MLMQ_E_INVALID;
+    HIP_TRY(c, hipSetDevice(c->device));
+    uint32_t *src = nullptr, *dst = nullptr;
+    HIP_TRY(c, hipMalloc(&src, nbytes));
+    hipError_t e = hipMalloc(&dst, nbytes);
+    if (e != hipSuccess) {
+        (void)hipFree(src);
+        return hip_fail(c, e, "hipMalloc");
+    }
+    const uint64_t words = nbytes / 4;
+    e = mlmq::launch_fill_u32(src, words, 1u, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_start, c->stream);
+    if (e == hipSuccess) e = mlmq::launch_copy_u32(dst, src, words, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_stop, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return hip_fail(c, e, "copy probe");
+    if (device_ms) *device_ms = ms;
     return MLMQ_OK;
 }
 

@@ -27,6 +27,7 @@ struct SolverLaunch {
     uint32_t min_read;
     uint32_t max_sleep;
     uint32_t pop_max;
+    uint32_t spill;
     uint64_t timeout_ticks;
     int grid_blocks;
     bool slf;

@@ -16,6 +16,7 @@
 // solve completes even if only part of the grid is resident.
 //
 // S = evidence/l3_supplement_20260921_v2/single_source/ (paths above are in S).
+#include "hop_block.h"
 #include "l1_slf.h"
 #include "solver_internal.h"
 
@@ -32,6 +33,7 @@ struct SolveArgs {
     uint32_t min_read;
     uint32_t max_sleep;     // idle back-off cap (x 256 cycles)
     uint32_t pop_max;       // records taken from L1 per batch (<= 64)
+    uint32_t spill;         // L1 spill threshold (<= kL1Cap - 7 * 64)
     uint64_t timeout_ticks;
 };
 
@@ -50,37 +52,40 @@ struct PhaseClock {
     }
 };
 
-// Relax all out-edges of the records in lanes [0, nin) (simple_process,
-// SS/SSSP/sssp_run.cu:143-413).  `in` holds the records' hop blocks.
-//
-//  * Stale filter (reference :174, process on <=): dist[v] is loaded at the
-//    start of the batch and only waited for just before the first atomicMin,
-//    so it overlaps the scan and the edge reads.
-//  * Load balance: edge slots are spread over the 64 lanes by a DPP scan of
-//    degrees; the owner of a slot is a DPP running maximum of start markers
-//    (lane + 1 written at each record's first slot), carried across chunks.
-//  * Edges of inline vertices come from LDS; heavier ones from the CSR.
-//  * A lane pushes (dst, nd) only if its atomicMin strictly lowered dst's
-//    distance (:211-243); dst's hop block is loaded in the same round trip as
-//    the atomic and travels with the pushed record, so the next hop starts
-//    without any graph load.
-template <bool kSLF, bool kProf>
-__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t bdeg, uint32_t bstart,
-                              uint32_t nin, const BlockRing &in, uint32_t *map,
-                              const BlockRing &ring, L1State &l1, WaveStats &st,
-                              PhaseClock<kProf> &clk) {
+__device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
+    const uint4 &q = b.q[w >> 2];
+    switch (w & 3) {
+    case 0: return q.x;
+    case 1: return q.y;
+    case 2: return q.z;
+    default: return q.w;
+    }
+}
+
+template <bool kSLF>
+__device__ __forceinline__ bool push_one(uint64_t *ring, L1State &l1, uint64_t rec, bool improved,
+                                         const SolveArgs &a, WaveStats &st) {
+    const uint64_t im = ballot(improved);
+    if (!im) return true;
+    if (kSLF) {
+        const uint64_t r[1] = {rec};
+        const uint64_t m[1] = {im};
+        return l1_push<1>(ring, l1, r, m, a.spill, a.q, st);
+    }
+    return l2_push(a.q, rec, improved, st);
+}
+
+// Heavy records (more out-edges than a hop block holds): edge slots spread
+// over the lanes by a DPP scan of degrees, the owner of a slot found as a DPP
+// running maximum of start markers, columns and weights read from the CSR.
+template <bool kSLF>
+__device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
+                            uint32_t hdeg, uint64_t *ring, L1State &l1, uint32_t *map,
+                            WaveStats &st) {
     const uint32_t lane = lane_id();
-    const bool valid = lane < nin;
-    const uint32_t v = rec_vertex(rec);
-    const uint32_t d = rec_dist(rec);
-    const uint32_t deg = valid ? bdeg : 0u;
-    const bool inl = deg <= kInlineEdges;
-    const uint32_t start_inl = bstart | (inl ? 0x80000000u : 0u);
-    uint32_t cur = 0;
-    if (valid) cur = ld_relaxed(a.dist + v);
+    const uint32_t deg = heavy ? hdeg : 0u;
     uint32_t total = 0;
     const uint32_t excl = wave_excl_scan(deg, total);
-    bool keep_known = false, keep = false;
     uint32_t carry = 0;
     bool ok = true;
     for (uint32_t base = 0; base < total && ok; base += kWave) {
@@ -94,75 +99,106 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t bdeg, u
         carry = read_lane(o1, kWave - 1);
         const uint32_t o = act ? o1 - 1 : 0u;
         const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)o, kWave);
-        const uint32_t o_si = (uint32_t)__shfl((int)start_inl, (int)o, kWave);
-        const uint32_t k = qi - o_excl;
-        st.v[kStatChunks] += 1;
-        clk.lap(st, kClkOwner);
-        uint2 cw = make_uint2(0u, 0u);
-        if (act) {
-            if (o_si >> 31) {
-                cw = block_edge_lds(in.blk, in.cap, o, k);
-            } else {
-                const uint32_t e = (o_si & 0x7FFFFFFFu) + k;
-                cw = make_uint2(a.col[e], a.wgt[e]);
-            }
-        }
-        if (!keep_known) {
-            keep = valid && d <= cur;
-            keep_known = true;
-            st.v[kStatStale] += popc(ballot(valid && !keep));
-        }
-        const uint32_t o_dk = (uint32_t)__shfl((int)(d | (keep ? 0x80000000u : 0u)), (int)o, kWave);
-        const bool go = act && (o_dk >> 31);
-        if (kProf) {
-            // make the lap wait for the edge reads and the stale check
-            if (ballot(go && cw.x == 0xFFFFFFFFu)) st.v[kStatSpare0] += 1;
-            clk.lap(st, kClkEdgeRead);
-        }
+        const uint32_t o_start = (uint32_t)__shfl((int)hstart, (int)o, kWave);
+        const uint32_t o_d = (uint32_t)__shfl((int)d, (int)o, kWave);
         bool improved = false;
-        uint32_t nd = 0;
-        HopBlock cb = empty_block();
-        if (go) {
-            nd = (o_dk & 0x7FFFFFFFu) + cw.y;
-            const uint32_t prev = fetch_min(a.dist + cw.x, nd);
-            cb = load_block(a.blocks, cw.x);
-            improved = nd < prev;
+        uint32_t c = 0, nd = 0;
+        if (act) {
+            const uint32_t e = o_start + (qi - o_excl);
+            c = a.col[e];
+            nd = o_d + a.wgt[e];
+            improved = nd < fetch_min(a.dist + c, nd);
         }
-        if (kProf) {
-            if (ballot(go && (cb.q[3].w == 0xFFFFFFFFu))) st.v[kStatSpare1] += 1;
-            clk.lap(st, kClkAtomic);
-        }
-        const uint64_t gm = ballot(go);
-        const uint64_t im = ballot(improved);
-        st.v[kStatRelax] += popc(gm);
-        st.v[kStatAtomic] += popc(gm);
-        st.v[kStatImprove] += popc(im);
-        clk.lap(st, kClkEdges);
-        if (im) {
-            const uint64_t nrec = make_rec(cw.x, nd);
-            if (kSLF) ok = l1_push(ring, l1, nrec, cb, im, a.q, st);
-            else ok = l2_push(a.q, nrec, improved, st);
-            clk.lap(st, kClkFlush);
-        }
+        st.v[kStatRelax] += popc(ballot(act));
+        st.v[kStatImprove] += popc(ballot(improved));
+        ok = push_one<kSLF>(ring, l1, make_rec(c, nd), improved, a, st);
     }
-    if (!keep_known) st.v[kStatStale] += popc(ballot(valid && d > cur));
+    return ok;
+}
+
+// Relax all out-edges of the records in lanes [0, nin) (simple_process,
+// SS/SSSP/sssp_run.cu:143-413), one record per lane, in two round trips:
+//   1. the record's current distance (stale filter, reference :174: process
+//      on <=) and its 64-byte hop block, loaded together;
+//   2. one agent-scope atomicMin per inline out-edge, all issued back to
+//      back (up to 7 per lane) before any result is used.
+// A (dst, nd) record is pushed only if its atomicMin strictly lowered dst's
+// distance (:211-243).  The batch's pushes go to L1 in one pass.
+template <bool kSLF, bool kProf>
+__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, uint64_t *ring,
+                              L1State &l1, uint32_t *map, WaveStats &st, PhaseClock<kProf> &clk) {
+    const uint32_t lane = lane_id();
+    const bool valid = lane < nin;
+    const uint32_t v = rec_vertex(rec);
+    const uint32_t d = rec_dist(rec);
+    uint32_t cur = 0;
+    HopBlock b = empty_block();
+    if (valid) {
+        cur = ld_relaxed(a.dist + v);
+        b = load_block(a.blocks, v);
+    }
+    const bool keep = valid && d <= cur;
+    st.v[kStatStale] += popc(ballot(valid && !keep));
+    const uint32_t deg = keep ? b.deg() : 0u;
+    const bool heavy = deg > kInlineEdges;
+    const uint32_t ideg = heavy ? 0u : deg;
+    clk.lap(st, kClkRow);
+
+    uint32_t c[kInlineEdges], nd[kInlineEdges], pv[kInlineEdges];
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) {
+        c[k] = word_of(b, 2 + 2 * k);
+        nd[k] = d + word_of(b, 3 + 2 * k);
+        pv[k] = 0;
+        if ((uint32_t)k < ideg) pv[k] = fetch_min(a.dist + c[k], nd[k]);
+    }
+    // Keep every result opaque until all atomics are issued; otherwise the
+    // compiler sinks each comparison into its predicated block and waits
+    // for the atomics one at a time.
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pv[k]));
+    uint64_t out[kInlineEdges], im[kInlineEdges];
+    uint32_t relax = 0, improved = 0;
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) {
+        const bool on = (uint32_t)k < ideg;
+        im[k] = ballot(on && nd[k] < pv[k]);
+        out[k] = make_rec(c[k], nd[k]);
+        relax += popc(ballot(on));
+        improved += popc(im[k]);
+    }
+    st.v[kStatRelax] += relax;
+    st.v[kStatImprove] += improved;
+    clk.lap(st, kClkAtomic);
+
+    bool ok = true;
+    if (improved) {
+        if (kSLF) {
+            ok = l1_push<kInlineEdges>(ring, l1, out, im, a.spill, a.q, st);
+        } else {
+#pragma unroll
+            for (int k = 0; k < (int)kInlineEdges; ++k)
+                if (ok && im[k]) ok = l2_push(a.q, out[k], (im[k] >> lane) & 1, st);
+        }
+        clk.lap(st, kClkFlush);
+    }
+    if (ok && ballot(heavy)) {
+        ok = relax_heavy<kSLF>(a, heavy, d, b.start(), deg, ring, l1, map, st);
+        clk.lap(st, kClkEdges);
+    }
     return ok;
 }
 
 template <int kWaves, bool kSLF, bool kProf>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
-    __shared__ uint64_t s_ring_rec[kWaves][kL1Cap];
-    __shared__ uint4 s_ring_blk[kWaves][4 * kL1Cap];
-    __shared__ uint64_t s_in_rec[kWaves][kWave];
-    __shared__ uint4 s_in_blk[kWaves][4 * kWave];
+    __shared__ uint64_t s_ring[kWaves][kL1Cap];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
 
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / kWave;
     const uint32_t gwave = blockIdx.x * kWaves + w;
-    const BlockRing ring{s_ring_rec[w], s_ring_blk[w], kL1Cap};
-    const BlockRing in{s_in_rec[w], s_in_blk[w], (uint32_t)kWave};
+    uint64_t *ring = s_ring[w];
     uint32_t *map = s_map[w];
     uint64_t *stage = s_stage[w];
     const L2Queue &q = a.q;
@@ -186,22 +222,17 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
 
     for (;;) {
         uint64_t rec = kEmpty;
-        HopBlock b = empty_block();
         uint32_t nin = 0;
         if (kSLF) {
-            nin = l1_pop(ring, l1, a.pop_max, rec, b);
+            nin = l1_pop(ring, l1, a.pop_max, rec);
             clk.lap(st, kClkL1Pop);
         }
         if (nin < a.min_read) {  // here L1 is empty
             const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, kWave - nin, st);
             if (got) {
                 credit += got;
-                if (lane >= nin && lane < nin + got) {
-                    rec = stage[lane - nin];
-                    b = load_block(a.blocks, rec_vertex(rec));
-                }
+                if (lane >= nin && lane < nin + got) rec = stage[lane - nin];
                 nin += got;
-                clk.lap(st, kClkRow);
             }
             clk.lap(st, kClkL2Pop);
         }
@@ -224,11 +255,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
         }
         sleep = 0;
         st.v[kStatBatches] += 1;
-        if (lane < nin) ring_put(in, lane, rec, b);
-        wave_sync();
-        if (!process_batch<kSLF, kProf>(a, rec, b.deg(), b.start(), nin, in, map, ring, l1, st,
-                                        clk))
-            break;
+        if (!process_batch<kSLF, kProf>(a, rec, nin, ring, l1, map, st, clk)) break;
         if (credit && l1.size == 0) {
             if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
             credit = 0;
@@ -243,6 +270,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
         }
     }
     if (kProf) st.v[kClkTotal] = clock64() - t_begin;
+    st.v[kStatAtomic] = st.v[kStatRelax];
 
     if (lane == 0) {
         uint64_t *stats = reinterpret_cast<uint64_t *>(q.ctl + kStatsOff);
@@ -380,6 +408,7 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     a.min_read = L.min_read;
     a.max_sleep = L.max_sleep;
     a.pop_max = L.pop_max;
+    a.spill = L.spill;
     a.timeout_ticks = L.timeout_ticks;
     const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
     if (L.slf && !L.profile)

@@ -120,16 +120,18 @@ class HostGraph:
 WORKLOADS = {
     # 264,346 points, 3-NN, weight = max(1, round(len * 1e5)), source 0, delta 200000
     "road264k": dict(kind="knn", n=264_346, k=3, seed=1, weight_scale=1e5, morton=False,
-                     source="zero", delta=200_000),
+                     source="zero", delta=200_000, delta_sweep=(200_000,)),
     # 6,262,104 points, 3-NN, Morton order, longest edge -> 100,000, centroid source
     "road6m": dict(kind="knn", n=6_262_104, k=3, seed=2, weight_scale=-100_000.0, morton=True,
-                   source="centroid", delta=200_000),
+                   source="centroid", delta=200_000,
+                   delta_sweep=(50_000, 100_000, 200_000, 400_000)),
     # RMAT scale 22, edge factor 16, (0.57, 0.19, 0.19, 0.05), weights [1, 1000]
     "rmat22": dict(kind="rmat", scale=22, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=3,
-                   wmin=1, wmax=1000, source="maxdeg", delta=200_000),
+                   wmin=1, wmax=1000, source="maxdeg", delta=200_000,
+                   delta_sweep=(1_000, 10_000, 200_000)),
     # 1,070,376 points, 3-NN, longest edge -> 50,000 (batched-query graph)
     "road1m": dict(kind="knn", n=1_070_376, k=3, seed=4, weight_scale=-50_000.0, morton=True,
-                   source="centroid", delta=200_000),
+                   source="centroid", delta=200_000, delta_sweep=(100_000, 200_000, 400_000)),
 }
 
 

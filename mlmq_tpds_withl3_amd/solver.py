@@ -107,6 +107,12 @@ class Solver:
               self._h)
         return {"blocks_per_cu_max": b.value, "cus": c.value, "threads_per_block": t.value}
 
+    def probe_copy(self, nbytes: int) -> float:
+        """Device ms of one plain copy kernel over `nbytes` (counter calibration)."""
+        ms = ctypes.c_double()
+        check(self._L.mlmq_probe_copy(self._h, int(nbytes), ctypes.byref(ms)), self._h)
+        return ms.value
+
     # -- queries --
     def solve(self, src: int, params: Params | None = None, **overrides) -> dict:
         p = params if params is not None else default_params(**overrides)
