@@ -56,7 +56,8 @@ def main():
     ap.add_argument("--queue", default="L1SLF_L2DQ")
     ap.add_argument("--work-blocks", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
-    ap.add_argument("--l1-pop", type=int, default=32)
+    ap.add_argument("--l1-pop", type=int, default=64)
+    ap.add_argument("--l1-donate", type=int, default=0)
     ap.add_argument("--l2-batch", type=int, default=8)
     ap.add_argument("--min-read", type=int, default=8)
     ap.add_argument("--buckets", type=int, default=16)
@@ -92,7 +93,8 @@ def main():
         return default_params(queue=args.queue, delta=delta, work_blocks=args.work_blocks,
                               blocks_per_cu=args.blocks_per_cu, l2_batch=args.l2_batch,
                               min_read=args.min_read, buckets=args.buckets,
-                              bucket_window=args.window, l1_pop=args.l1_pop, timeout_s=30.0)
+                              bucket_window=args.window, l1_pop=args.l1_pop,
+                              l1_donate=args.l1_donate, timeout_s=30.0)
 
     solver = Solver(0)
     solver.load_csr(row, col, w)
@@ -173,7 +175,7 @@ def main():
             "queue": args.queue, "delta": delta,
             "delta_sweep_device_ms": {str(k): round(v, 4) for k, v in sweep.items()},
             "buckets": args.buckets, "bucket_window": args.window,
-            "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop,
+            "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop, "l1_donate": args.l1_donate,
             "work_blocks": last["work_blocks"], "waves": last["waves"],
             "cus": shape["cus"], "blocks_per_cu_max": shape["blocks_per_cu_max"],
         },
@@ -184,7 +186,7 @@ def main():
         "work": {"relaxations": last["relaxations"], "atomics": last["atomics"],
                  "improvements": last["improvements"], "l2_pushes": last["l2_pushes"],
                  "l2_pops": last["l2_pops"], "stale": last["stale"],
-                 "batches": last["batches"],
+                 "batches": last["batches"], "donated": last["donated"],
                  "inflation": round(last["relaxations"] / max(1, e_r), 4)},
         "correct": correct,
         "roofline": {

@@ -67,7 +67,7 @@ typedef struct mlmq_params {
     int32_t  queue_mode;      /* MLMQ_QUEUE_L1SLF_L2DQ (default) or MLMQ_QUEUE_L1N_L2DQ */
     uint32_t delta;           /* bucket width; reference BENCH_DELTA, default 200000 */
     int32_t  work_blocks;     /* persistent workgroups; 0 = automatic (blocks_per_cu x CUs if
-                                 blocks_per_cu > 0, else n / 25000 clamped to [32, 256]) */
+                                 blocks_per_cu > 0, else n / 100000 clamped to [32, 128]) */
     int32_t  blocks_per_cu;   /* used when work_blocks == 0 */
     int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
     int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */
@@ -80,7 +80,9 @@ typedef struct mlmq_params {
     int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 32 */
     int32_t  l1_spill;        /* L1 holding more records than this after a push spills to L2;
                                  1..64, default 64 */
-    int32_t  reserved[4];
+    int32_t  l1_donate;       /* while some waves are idle, a wave whose L1 holds more than this
+                                 moves the back half to L2; 0 = never (default) */
+    int32_t  reserved[3];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -109,7 +111,7 @@ typedef struct mlmq_timing {
      * the stale check wait), atomicMin round trip; 16 empty laps (clock
      * overhead); edge chunks processed */
     uint64_t detail_cycles[5];
-    uint64_t reserved[1];
+    uint64_t donated;         /* records moved from L1 to L2 for idle waves */
 } mlmq_timing;
 
 /* Reference graph statistics struct (core/include/graph_info.h:8-20). */

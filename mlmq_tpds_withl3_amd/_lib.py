@@ -51,7 +51,7 @@ class Params(ctypes.Structure):
         ("bucket_capacity", ctypes.c_int64), ("timeout_s", ctypes.c_double),
         ("auto_retry", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("l1_pop", ctypes.c_int32), ("l1_spill", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("l1_donate", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -66,7 +66,7 @@ class Timing(ctypes.Structure):
         ("improvements", ctypes.c_uint64), ("stale", ctypes.c_uint64),
         ("batches", ctypes.c_uint64), ("idle_polls", ctypes.c_uint64),
         ("retries", ctypes.c_uint64), ("phase_cycles", ctypes.c_uint64 * 8),
-        ("detail_cycles", ctypes.c_uint64 * 5), ("reserved", ctypes.c_uint64 * 1),
+        ("detail_cycles", ctypes.c_uint64 * 5), ("donated", ctypes.c_uint64),
     ]
     DETAILS = ("owner", "edge_read", "atomic", "laps16", "chunks")
     PHASES = ("l1_pop", "l2_pop", "row_fetch", "edges", "flush", "idle", "total", "credit")

@@ -29,7 +29,8 @@ constexpr uint32_t kBaseOff = kTakenOff + kMaxBuckets * kStride;  // window star
 constexpr uint32_t kDoneOff = kBaseOff + kStride;                 // credits returned
 constexpr uint32_t kExitOff = kDoneOff + kStride;                 // 1 = stop
 constexpr uint32_t kErrOff = kExitOff + kStride;                  // MLMQ device error code
-constexpr uint32_t kStatsOff = kErrOff + kStride;                 // uint64 stats[kNumStats]
+constexpr uint32_t kHungryOff = kErrOff + kStride;                // waves currently without work
+constexpr uint32_t kStatsOff = kHungryOff + kStride;              // uint64 stats[kNumStats]
 constexpr uint32_t kNumStats = 24;
 constexpr uint32_t kCtlWords = kStatsOff + 2 * kNumStats + kStride;
 
@@ -39,11 +40,11 @@ enum StatIndex : uint32_t {
     kStatStale, kStatBatches, kStatIdle,
     // per-phase clock totals (profiling build of the kernel only)
     kClkL1Pop, kClkL2Pop, kClkRow, kClkEdges, kClkFlush, kClkIdle, kClkTotal, kClkCredit,
-    kClkOwner, kClkEdgeRead, kClkAtomic, kClkLaps, kStatChunks, kStatSpare0, kStatSpare1, kStatSpare2
+    kClkOwner, kClkEdgeRead, kClkAtomic, kClkLaps, kStatChunks, kStatDonated, kStatSpare1, kStatSpare2
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return __lanemask_lt(); }
 __device__ __forceinline__ uint32_t popc(uint64_t x) { return (uint32_t)__popcll(x); }
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }

@@ -29,10 +29,18 @@ struct L1State {
     uint32_t size;
 };
 
+// Head and size are wave-uniform; re-assert it so the compiler keeps them in
+// scalar registers and branches on them without exec masking.
+__device__ __forceinline__ void l1_uniform(L1State &l1) {
+    l1.head = uniform(l1.head);
+    l1.size = uniform(l1.size);
+}
+
 // Move up to `maxn` (<= 64) records from the front into lanes [0, k).
 __device__ __forceinline__ uint32_t l1_pop(const uint64_t *ring, L1State &l1, uint32_t maxn,
                                            uint64_t &rec) {
-    const uint32_t k = l1.size < maxn ? l1.size : maxn;
+    l1_uniform(l1);
+    const uint32_t k = uniform(l1.size < maxn ? l1.size : maxn);
     if (lane_id() < k) rec = ring[(l1.head + lane_id()) & kL1Mask];
     wave_sync();
     l1.head = (l1.head + k) & kL1Mask;
@@ -52,6 +60,23 @@ __device__ bool l1_spill_all(const uint64_t *ring, L1State &l1, const L2Queue &q
     return ok;
 }
 
+// Work donation (no reference counterpart): move up to `want` records from
+// the back of the deque -- the ones SLF would process last -- to L2, where
+// idle waves can claim them.  Returns false if L2 overflowed.
+__device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, const L2Queue &q,
+                          WaveStats &st) {
+    l1_uniform(l1);
+    const uint32_t m = uniform(want < l1.size ? want : l1.size);
+    if (m == 0) return true;
+    const uint32_t k = m < (uint32_t)kWave ? m : (uint32_t)kWave;
+    uint64_t x = kEmpty;
+    if (lane_id() < k) x = ring[(l1.head + l1.size - k + lane_id()) & kL1Mask];
+    wave_sync();
+    l1.size -= k;
+    st.v[kStatDonated] += k;
+    return l2_push(q, x, lane_id() < k, st);
+}
+
 // Push K registers of records: record k of lane i is present when bit i of
 // mask[k] is set; order is (k, lane).  Requires size <= spill on entry and
 // spill + K * 64 <= kL1Cap.
@@ -59,19 +84,26 @@ template <int K>
 __device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
                         const uint64_t (&mask)[K], uint32_t spill, const L2Queue &q,
                         WaveStats &st) {
+    l1_uniform(l1);
     uint32_t total = 0;
-    uint32_t pivot = 0;
-    bool have_pivot = l1.size != 0;
-    if (have_pivot) pivot = uniform(rec_dist(ring[l1.head]));
+    // Pivot: the head's distance, or on an empty deque the first pushed
+    // record's (first register with a record, lowest lane).
+    int first_k = K;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (!have_pivot && mask[k]) {
-            pivot = read_lane(rec_dist(rec[k]), (uint32_t)__builtin_ctzll(mask[k]));
-            have_pivot = true;
-        }
+    for (int k = K - 1; k >= 0; --k) {
+        if (mask[k]) first_k = k;
         total += popc(mask[k]);
     }
     if (total == 0) return true;
+    uint32_t pivot = 0;
+    if (l1.size) {
+        pivot = uniform(rec_dist(ring[l1.head]));
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k == first_k)
+                pivot = read_lane(rec_dist(rec[k]), (uint32_t)__builtin_ctzll(mask[k]));
+    }
     const uint32_t lane = lane_id();
     const uint64_t lt = lanemask_lt();
     uint32_t nf = 0, nb = 0;

@@ -231,8 +231,9 @@ void mlmq_params_default(mlmq_params *p) {
     p->timeout_s = 60.0;
     p->auto_retry = 1;
     p->profile = 0;
-    p->l1_pop = 32;
+    p->l1_pop = 64;
     p->l1_spill = 64;
+    p->l1_donate = 0;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -406,6 +407,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
     if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
     if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
+    if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -418,13 +420,13 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         const int per_cu = p.blocks_per_cu < occ ? p.blocks_per_cu : occ;
         blocks = per_cu * c->cus;
     } else if (blocks == 0) {
-        // Latency-bound solver: one wave per ~100 frontier vertices is enough.
-        // A road-like graph of n vertices keeps O(sqrt(n)) to O(n / 1000)
-        // records in flight; more waves only add queue polling.  Measured
-        // optimum: ~32 workgroups at 0.26 M vertices, ~256 at 6.3 M.
-        blocks = c->n / 25000;
+        // Latency-bound solver: a road-like graph of n vertices keeps only
+        // O(sqrt(n)) records in flight, so more waves mostly add queue
+        // polling.  Measured optimum on MI355X: ~32 workgroups (128 waves)
+        // at 0.26 M vertices, ~64 at 6.3 M.
+        blocks = c->n / 100000;
         if (blocks < 32) blocks = 32;
-        if (blocks > 256) blocks = 256;
+        if (blocks > 128) blocks = 128;
     }
     if (blocks > occ * c->cus) blocks = occ * c->cus;
 
@@ -466,6 +468,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.max_sleep = 16;
         L.pop_max = (uint32_t)p.l1_pop;
         L.spill = (uint32_t)p.l1_spill;
+        L.donate_min = (uint32_t)p.l1_donate;
         L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
         L.grid_blocks = blocks;
         L.slf = slf;
@@ -531,6 +534,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
             timing->retries = retries;
             for (int i = 0; i < 8; ++i) timing->phase_cycles[i] = st[mlmq::kClkL1Pop + i];
             for (int i = 0; i < 5; ++i) timing->detail_cycles[i] = st[mlmq::kClkOwner + i];
+            timing->donated = st[mlmq::kStatDonated];
         }
         return MLMQ_OK;
     }

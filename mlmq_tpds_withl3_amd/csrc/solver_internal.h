@@ -28,6 +28,7 @@ struct SolverLaunch {
     uint32_t max_sleep;
     uint32_t pop_max;
     uint32_t spill;
+    uint32_t donate_min;
     uint64_t timeout_ticks;
     int grid_blocks;
     bool slf;

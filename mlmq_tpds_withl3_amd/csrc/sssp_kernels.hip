@@ -34,6 +34,7 @@ struct SolveArgs {
     uint32_t max_sleep;     // idle back-off cap (x 256 cycles)
     uint32_t pop_max;       // records taken from L1 per batch (<= 64)
     uint32_t spill;         // L1 spill threshold (<= kL1Cap - 7 * 64)
+    uint32_t donate_min;    // donate to L2 while waves are idle and L1 holds more (0 = off)
     uint64_t timeout_ticks;
 };
 
@@ -126,7 +127,8 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
 // distance (:211-243).  The batch's pushes go to L1 in one pass.
 template <bool kSLF, bool kProf>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, uint64_t *ring,
-                              L1State &l1, uint32_t *map, WaveStats &st, PhaseClock<kProf> &clk) {
+                              L1State &l1, uint32_t *map, uint32_t (&pf)[kInlineEdges],
+                              uint32_t &hungry, WaveStats &st, PhaseClock<kProf> &clk) {
     const uint32_t lane = lane_id();
     const bool valid = lane < nin;
     const uint32_t v = rec_vertex(rec);
@@ -137,6 +139,8 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
         cur = ld_relaxed(a.dist + v);
         b = load_block(a.blocks, v);
     }
+    // idle-wave count for work donation, in the same round trip
+    hungry = a.donate_min ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
     const bool keep = valid && d <= cur;
     st.v[kStatStale] += popc(ballot(valid && !keep));
     const uint32_t deg = keep ? b.deg() : 0u;
@@ -149,6 +153,20 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         c[k] = word_of(b, 2 + 2 * k);
         nd[k] = d + word_of(b, 3 + 2 * k);
+    }
+    // Prefetch, in the same round trip as the atomics: touch every relaxed
+    // child's hop block, so the batch that later pops the child finds the
+    // line in this XCD's L2 (its dist line is already there from the
+    // atomic).  Unpredicated -- lanes without an edge touch vertex 0's
+    // block -- so there is no branch and no wait; the loaded words are
+    // consumed one batch later.
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) {
+        const uint32_t t = (uint32_t)k < ideg ? c[k] : 0u;
+        pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
+    }
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) {
         pv[k] = 0;
         if ((uint32_t)k < ideg) pv[k] = fetch_min(a.dist + c[k], nd[k]);
     }
@@ -219,6 +237,11 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     uint32_t cl_start = 0;      // lane b: start of the claim on bucket b
     uint64_t cl_mask = 0;       // lane b: pending slots of that claim
     uint32_t sleep = 0;
+    uint32_t pf[kInlineEdges], pf_prev[kInlineEdges];   // prefetch results, consumed one batch late
+    uint32_t sunk = 0;
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) pf[k] = pf_prev[k] = 0;
+    bool idle_listed = false;   // counted in the control block's hungry-wave count
 
     for (;;) {
         uint64_t rec = kEmpty;
@@ -241,6 +264,10 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
                 if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
                 credit = 0;
             }
+            if (a.donate_min && !idle_listed) {
+                if (lane == 0) fetch_add(q.ctl + kHungryOff, 1u);
+                idle_listed = true;
+            }
             st.v[kStatIdle] += 1;
             if (gwave == 0 && l2_check_done(q)) break;
             if (uniform(ld_relaxed(q.ctl + kExitOff))) break;
@@ -254,8 +281,23 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
             continue;
         }
         sleep = 0;
+        if (idle_listed) {
+            if (lane == 0) fetch_add(q.ctl + kHungryOff, 0xFFFFFFFFu);
+            idle_listed = false;
+        }
         st.v[kStatBatches] += 1;
-        if (!process_batch<kSLF, kProf>(a, rec, nin, ring, l1, map, st, clk)) break;
+        uint32_t hungry = 0;
+        if (!process_batch<kSLF, kProf>(a, rec, nin, ring, l1, map, pf, hungry, st, clk)) break;
+        if (kSLF && hungry && l1.size > a.donate_min) {
+            // (uniform: hungry is a scalar load result)
+            if (!l1_donate(ring, l1, uniform(l1.size) / 2, q, st)) break;
+            clk.lap(st, kClkFlush);
+        }
+#pragma unroll
+        for (int k = 0; k < (int)kInlineEdges; ++k) {
+            sunk ^= pf_prev[k];
+            pf_prev[k] = pf[k];
+        }
         if (credit && l1.size == 0) {
             if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
             credit = 0;
@@ -271,6 +313,9 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     }
     if (kProf) st.v[kClkTotal] = clock64() - t_begin;
     st.v[kStatAtomic] = st.v[kStatRelax];
+#pragma unroll
+    for (int k = 0; k < (int)kInlineEdges; ++k) sunk ^= pf_prev[k];
+    st.v[kStatSpare2] += popc(ballot(sunk == 0x5EEDF00Du));
 
     if (lane == 0) {
         uint64_t *stats = reinterpret_cast<uint64_t *>(q.ctl + kStatsOff);
@@ -409,6 +454,7 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     a.max_sleep = L.max_sleep;
     a.pop_max = L.pop_max;
     a.spill = L.spill;
+    a.donate_min = L.donate_min;
     a.timeout_ticks = L.timeout_ticks;
     const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
     if (L.slf && !L.profile)

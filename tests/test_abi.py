@@ -44,7 +44,7 @@ def test_loader_binds_all():
     L.mlmq_params_default(ctypes.byref(p))
     assert (p.queue_mode, p.delta, p.buckets, p.bucket_window, p.l2_batch, p.min_read) == \
         (19, 200000, 16, 4, 8, 8)
-    assert (p.l1_pop, p.l1_spill, p.profile) == (32, 64, 0)
+    assert (p.l1_pop, p.l1_spill, p.l1_donate, p.profile) == (64, 64, 0, 0)
 
 
 def test_library_is_gfx950_and_independent_of_oracle():
