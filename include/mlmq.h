@@ -82,7 +82,11 @@ typedef struct mlmq_params {
                                  1..64, default 64 */
     int32_t  l1_donate;       /* while some waves are idle, a wave whose L1 holds more than this
                                  moves the back half to L2; 0 = never (default) */
-    int32_t  reserved[3];
+    int32_t  batch_waves;     /* mlmq_solve_batch: waves per query (workgroup size), 4, 8 or 16;
+                                 default 8 */
+    int32_t  use_graph;       /* mlmq_solve_batch: replay the launch as a captured hipGraph
+                                 (default 1) */
+    int32_t  reserved[1];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -113,6 +117,29 @@ typedef struct mlmq_timing {
     uint64_t detail_cycles[5];
     uint64_t donated;         /* records moved from L1 to L2 for idle waves */
 } mlmq_timing;
+
+/* Statistics of one mlmq_solve_batch call. */
+typedef struct mlmq_batch_timing {
+    double   solve_ms;        /* host steady clock around the launch (row reset + batch kernel) */
+    double   device_ms;       /* HIP events around the launch */
+    double   query_ms_p50;    /* per-query time inside the launch (device wall clock), median */
+    double   query_ms_p99;
+    double   query_ms_max;
+    double   query_ms_mean;
+    int32_t  queries;
+    int32_t  work_blocks;     /* workgroups = queries in flight */
+    int32_t  waves_per_query;
+    int32_t  graph_replay;    /* 1 if the launch was a hipGraph replay */
+    int64_t  bucket_capacity;
+    uint64_t relaxations;
+    uint64_t improvements;
+    uint64_t l2_pushes;
+    uint64_t stale;
+    uint64_t batches;
+    uint64_t idle_polls;
+    uint64_t retries;
+    uint64_t reserved[8];
+} mlmq_batch_timing;
 
 /* Reference graph statistics struct (core/include/graph_info.h:8-20). */
 typedef struct mlmq_graph_info {
@@ -145,6 +172,18 @@ int mlmq_graph_size(const mlmq_ctx *ctx, int32_t *n, int64_t *m);
 /* One query: reset, seed `src`, run the persistent solver to termination.
  * `params` may be NULL (defaults); `timing` may be NULL. */
 int mlmq_solve(mlmq_ctx *ctx, int32_t src, const mlmq_params *params, mlmq_timing *timing);
+/* Many queries in one persistent launch (no reference counterpart in the
+ * single-GPU solver; the reference answers a source list with one solve
+ * after another, SSSP/main.cu:213-307).  Each workgroup of
+ * params->batch_waves waves owns a private queue partition and serves queries
+ * taken from a global counter; params->work_blocks workgroups are in flight
+ * (0 = one per CU, at most nq).  Results stay on the device until copied
+ * with mlmq_copy_batch_dist. */
+int mlmq_solve_batch(mlmq_ctx *ctx, const int32_t *sources, int32_t nq,
+                     const mlmq_params *params, mlmq_batch_timing *timing);
+/* Copy the n distances of query `query` of the last batch (caller-owned). */
+int mlmq_copy_batch_dist(mlmq_ctx *ctx, int32_t query, int32_t *host_out);
+
 /* Copy the n distances of the last solve into host memory (caller-owned). */
 int mlmq_copy_dist(mlmq_ctx *ctx, int32_t *host_out);
 /* Device pointer to the distance array of the last solve (n int32). */

@@ -26,7 +26,7 @@ SOLVER_SYMBOLS = [
     "mlmq_params_default", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
     "mlmq_load_csr", "mlmq_load_gr", "mlmq_attach_device_csr", "mlmq_graph_stats",
     "mlmq_graph_size", "mlmq_solve", "mlmq_copy_dist", "mlmq_dist_device_ptr",
-    "mlmq_device_shape", "mlmq_probe_copy", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
+    "mlmq_device_shape", "mlmq_probe_copy", "mlmq_solve_batch", "mlmq_copy_batch_dist", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
 ]
 GRAPH_SYMBOLS = [
     "mlmq_graph_generate_knn", "mlmq_graph_generate_rmat", "mlmq_graph_read_gr",
@@ -51,7 +51,8 @@ class Params(ctypes.Structure):
         ("bucket_capacity", ctypes.c_int64), ("timeout_s", ctypes.c_double),
         ("auto_retry", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("l1_pop", ctypes.c_int32), ("l1_spill", ctypes.c_int32),
-        ("l1_donate", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("l1_donate", ctypes.c_int32), ("batch_waves", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -77,6 +78,24 @@ class Timing(ctypes.Structure):
         d["phase_cycles"] = dict(zip(self.PHASES, list(self.phase_cycles)))
         d["detail_cycles"] = dict(zip(self.DETAILS, list(self.detail_cycles)))
         return d
+
+
+class BatchTiming(ctypes.Structure):
+    _fields_ = [
+        ("solve_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+        ("query_ms_p50", ctypes.c_double), ("query_ms_p99", ctypes.c_double),
+        ("query_ms_max", ctypes.c_double), ("query_ms_mean", ctypes.c_double),
+        ("queries", ctypes.c_int32), ("work_blocks", ctypes.c_int32),
+        ("waves_per_query", ctypes.c_int32), ("graph_replay", ctypes.c_int32),
+        ("bucket_capacity", ctypes.c_int64),
+        ("relaxations", ctypes.c_uint64), ("improvements", ctypes.c_uint64),
+        ("l2_pushes", ctypes.c_uint64), ("stale", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64), ("idle_polls", ctypes.c_uint64),
+        ("retries", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class GraphInfo(ctypes.Structure):
@@ -120,6 +139,9 @@ def load() -> ctypes.CDLL:
         "mlmq_device_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                              ctypes.POINTER(i32)]),
         "mlmq_probe_copy": (ctypes.c_int, [P, u64, ctypes.POINTER(ctypes.c_double)]),
+        "mlmq_solve_batch": (ctypes.c_int, [P, P, i32, ctypes.POINTER(Params),
+                                            ctypes.POINTER(BatchTiming)]),
+        "mlmq_copy_batch_dist": (ctypes.c_int, [P, i32, P]),
         "sssp_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P, P, P]),
         "sssp_run_adaptive": (ctypes.c_int, [ctypes.c_int, GraphInfo]),
         "sssp_cp_data": (ctypes.c_int, [ctypes.POINTER(ctypes.POINTER(ctypes.c_int))]),

@@ -34,7 +34,7 @@ constexpr uint32_t kStatsOff = kHungryOff + kStride;              // uint64 stat
 constexpr uint32_t kNumStats = 24;
 constexpr uint32_t kCtlWords = kStatsOff + 2 * kNumStats + kStride;
 
-enum DeviceError : uint32_t { kErrNone = 0, kErrOverflow = 1, kErrTimeout = 2 };
+enum DeviceError : uint32_t { kErrNone = 0, kErrOverflow = 1, kErrTimeout = 2, kErrInternal = 3 };
 enum StatIndex : uint32_t {
     kStatL2Push = 0, kStatL2Pop, kStatRelax, kStatAtomic, kStatImprove,
     kStatStale, kStatBatches, kStatIdle,

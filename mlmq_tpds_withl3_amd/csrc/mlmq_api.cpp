@@ -7,6 +7,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -40,6 +41,24 @@ struct mlmq_ctx {
     bool slots_dirty = true;
     bool solved = false;
     std::vector<uint32_t> ctl_host;
+    // batch launch
+    uint32_t *b_dist = nullptr;
+    uint64_t b_dist_words = 0;
+    int32_t b_nq = 0;
+    uint32_t *b_ctl = nullptr;
+    int b_ctl_blocks = 0;
+    uint64_t *b_slots = nullptr;
+    uint64_t b_slots_count = 0;
+    bool b_slots_dirty = true;
+    uint32_t *b_sources = nullptr;
+    int32_t b_sources_n = 0;
+    uint32_t *b_gctl = nullptr;
+    uint32_t *b_qstatus = nullptr;
+    uint64_t *b_qtime = nullptr;
+    int32_t b_q_alloc = 0;
+    bool b_solved = false;
+    hipGraphExec_t b_graph = nullptr;
+    std::vector<uint64_t> b_graph_key;
     std::string err;
 };
 
@@ -67,7 +86,28 @@ int hip_fail(mlmq_ctx *c, hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
     } while (0)
 
+void free_batch(mlmq_ctx *c) {
+    if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
+    c->b_graph = nullptr;
+    c->b_graph_key.clear();
+    for (void *p : {(void *)c->b_dist, (void *)c->b_ctl, (void *)c->b_slots, (void *)c->b_sources,
+                    (void *)c->b_gctl, (void *)c->b_qstatus, (void *)c->b_qtime})
+        if (p) (void)hipFree(p);
+    c->b_dist = nullptr;
+    c->b_ctl = nullptr;
+    c->b_slots = nullptr;
+    c->b_sources = nullptr;
+    c->b_gctl = nullptr;
+    c->b_qstatus = nullptr;
+    c->b_qtime = nullptr;
+    c->b_dist_words = c->b_slots_count = 0;
+    c->b_ctl_blocks = c->b_sources_n = c->b_q_alloc = c->b_nq = 0;
+    c->b_solved = false;
+    c->b_slots_dirty = true;
+}
+
 void free_graph(mlmq_ctx *c) {
+    free_batch(c);
     if (c->owns_graph) {
         (void)hipFree(c->row);
         (void)hipFree(c->col);
@@ -234,6 +274,8 @@ void mlmq_params_default(mlmq_params *p) {
     p->l1_pop = 64;
     p->l1_spill = 64;
     p->l1_donate = 0;
+    p->batch_waves = 8;
+    p->use_graph = 1;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -272,6 +314,7 @@ int mlmq_destroy(mlmq_ctx *c) {
     if (c->dist) (void)hipFree(c->dist);
     if (c->ctl) (void)hipFree(c->ctl);
     if (c->slots) (void)hipFree(c->slots);
+    free_batch(c);
     if (c->ev_reset) (void)hipEventDestroy(c->ev_reset);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
@@ -538,6 +581,236 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         }
         return MLMQ_OK;
     }
+}
+
+int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq_params *params,
+                     mlmq_batch_timing *timing) {
+    if (!c) return MLMQ_E_INVALID;
+    if (!c->row) return fail(c, MLMQ_E_NOGRAPH, "no graph loaded");
+    if (!sources || nq <= 0) return fail(c, MLMQ_E_INVALID, "empty source list");
+    mlmq_params p;
+    if (params) p = *params;
+    else mlmq_params_default(&p);
+    if (p.queue_mode != MLMQ_QUEUE_L1SLF_L2DQ)
+        return fail(c, MLMQ_E_UNSUPPORTED, "batched solves use L1SLF_L2DQ");
+    if (p.delta == 0 || p.delta > 0x7FFFFFFFu) return fail(c, MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
+    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return fail(c, MLMQ_E_INVALID, "buckets must be in [1, 64]");
+    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(c, MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
+    if (p.l2_batch < 1 || p.l2_batch > 64) return fail(c, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
+    if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
+    if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
+    if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
+    if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
+    if (p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
+        return fail(c, MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
+    if (p.work_blocks < 0 || p.bucket_capacity < 0) return fail(c, MLMQ_E_INVALID, "negative size parameter");
+    for (int32_t i = 0; i < nq; ++i)
+        if (sources[i] < 0 || sources[i] >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+
+    int occ = 0;
+    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, &occ));
+    if (occ < 1) return fail(c, MLMQ_E_HIP, "batch kernel cannot be resident");
+    int blocks = p.work_blocks ? p.work_blocks : c->cus;
+    if (blocks > occ * c->cus) blocks = occ * c->cus;
+    if (blocks > nq) blocks = nq;
+
+    uint64_t cap = (uint64_t)p.bucket_capacity;
+    if (cap == 0) cap = (uint64_t)c->n / 8 > 8192 ? (uint64_t)c->n / 8 : 8192;
+    uint32_t cap_log2 = ceil_log2(cap);
+    if (cap_log2 > 31) return fail(c, MLMQ_E_INVALID, "bucket capacity too large");
+
+    // per-query rows and bookkeeping
+    const uint64_t words = (uint64_t)nq * (uint64_t)c->n;
+    if (c->b_dist_words < words) {
+        if (c->b_dist) (void)hipFree(c->b_dist);
+        c->b_dist = nullptr;
+        c->b_dist_words = 0;
+        HIP_TRY(c, hipMalloc(&c->b_dist, sizeof(uint32_t) * words));
+        c->b_dist_words = words;
+    }
+    if (c->b_q_alloc < nq) {
+        for (void *q : {(void *)c->b_sources, (void *)c->b_qstatus, (void *)c->b_qtime})
+            if (q) (void)hipFree(q);
+        c->b_sources = c->b_qstatus = nullptr;
+        c->b_qtime = nullptr;
+        c->b_q_alloc = 0;
+        HIP_TRY(c, hipMalloc(&c->b_sources, sizeof(uint32_t) * nq));
+        HIP_TRY(c, hipMalloc(&c->b_qstatus, sizeof(uint32_t) * nq));
+        HIP_TRY(c, hipMalloc(&c->b_qtime, sizeof(uint64_t) * 2 * nq));
+        c->b_q_alloc = nq;
+    }
+    if (!c->b_gctl) HIP_TRY(c, hipMalloc(&c->b_gctl, sizeof(uint32_t) * mlmq::kCtlWords));
+    HIP_TRY(c, hipMemcpy(c->b_sources, sources, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
+
+    const double timeout_s = p.timeout_s > 0 ? p.timeout_s : 60.0;
+    uint64_t retries = 0;
+    for (;;) {
+        // queue partitions: one control block and bucket set per workgroup
+        if (c->b_ctl_blocks < blocks) {
+            if (c->b_ctl) (void)hipFree(c->b_ctl);
+            c->b_ctl = nullptr;
+            c->b_ctl_blocks = 0;
+            HIP_TRY(c, hipMalloc(&c->b_ctl, sizeof(uint32_t) * mlmq::kCtlWords * (size_t)blocks));
+            c->b_ctl_blocks = blocks;
+        }
+        const uint64_t need = (uint64_t)blocks * ((uint64_t)p.buckets << cap_log2);
+        if (c->b_slots_count < need) {
+            if (c->b_slots) (void)hipFree(c->b_slots);
+            c->b_slots = nullptr;
+            c->b_slots_count = 0;
+            HIP_TRY(c, hipMalloc(&c->b_slots, sizeof(uint64_t) * need));
+            c->b_slots_count = need;
+            c->b_slots_dirty = true;
+        }
+        if (c->b_slots_dirty) {
+            HIP_TRY(c, mlmq::launch_fill_u64(c->b_slots, c->b_slots_count, mlmq::kEmpty, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            c->b_slots_dirty = false;
+        }
+
+        mlmq::SolverLaunch L;
+        L.row = c->row;
+        L.col = c->col;
+        L.wgt = c->wgt;
+        L.blocks = c->blocks;
+        L.dist = nullptr;
+        L.slots = nullptr;
+        L.ctl = nullptr;
+        L.n = (uint32_t)c->n;
+        L.nbuckets = (uint32_t)p.buckets;
+        L.window = (uint32_t)p.bucket_window;
+        L.cap_log2 = cap_log2;
+        L.delta = p.delta;
+        L.batch = (uint32_t)p.l2_batch;
+        L.spin_limit = 1u << 22;
+        L.min_read = (uint32_t)p.min_read;
+        L.max_sleep = 16;
+        L.pop_max = (uint32_t)p.l1_pop;
+        L.spill = (uint32_t)p.l1_spill;
+        L.donate_min = (uint32_t)p.l1_donate;
+        L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
+        L.grid_blocks = blocks;
+        L.slf = true;
+        L.profile = false;
+        mlmq::BatchLaunch B;
+        B.ctl_all = c->b_ctl;
+        B.slots_all = c->b_slots;
+        B.dist_all = c->b_dist;
+        B.sources = c->b_sources;
+        B.nq = (uint32_t)nq;
+        B.gctl = c->b_gctl;
+        B.qstatus = c->b_qstatus;
+        B.qtime = c->b_qtime;
+        B.waves = p.batch_waves;
+
+        // The launch (row reset + batch kernel) is captured once per shape and
+        // replayed as a hipGraph; the source list is copied in before it.
+        const std::vector<uint64_t> key = {
+            (uint64_t)nq, (uint64_t)blocks, (uint64_t)p.batch_waves, cap_log2, p.delta,
+            (uint64_t)p.buckets, (uint64_t)p.bucket_window, (uint64_t)p.l2_batch,
+            (uint64_t)p.min_read, (uint64_t)p.l1_pop, (uint64_t)p.l1_spill, (uint64_t)p.l1_donate,
+            L.timeout_ticks, (uint64_t)(uintptr_t)c->b_dist, (uint64_t)(uintptr_t)c->b_slots,
+            (uint64_t)(uintptr_t)c->b_ctl, (uint64_t)(uintptr_t)c->b_sources,
+            (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks};
+        const bool use_graph = p.use_graph != 0;
+        if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
+            if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
+            c->b_graph = nullptr;
+            hipGraph_t g = nullptr;
+            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = mlmq::launch_batch_reset(c->b_dist, (uint32_t)c->n, (uint32_t)nq,
+                                                     c->b_sources, c->b_gctl, c->stream);
+            hipError_t e2 = mlmq::launch_batch(L, B, c->stream);
+            hipError_t e3 = hipStreamEndCapture(c->stream, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hip_fail(c, e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3, "batch graph capture");
+            }
+            hipError_t e4 = hipGraphInstantiate(&c->b_graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e4 != hipSuccess) {
+                c->b_graph = nullptr;
+                return hip_fail(c, e4, "hipGraphInstantiate");
+            }
+            c->b_graph_key = key;
+        }
+        const auto s0 = clk::now();
+        HIP_TRY(c, hipEventRecord(c->ev_start, c->stream));
+        if (use_graph) {
+            HIP_TRY(c, hipGraphLaunch(c->b_graph, c->stream));
+        } else {
+            HIP_TRY(c, mlmq::launch_batch_reset(c->b_dist, (uint32_t)c->n, (uint32_t)nq, c->b_sources,
+                                                c->b_gctl, c->stream));
+            HIP_TRY(c, mlmq::launch_batch(L, B, c->stream));
+        }
+        HIP_TRY(c, hipEventRecord(c->ev_stop, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const auto s1 = clk::now();
+
+        std::vector<uint32_t> g(mlmq::kCtlWords), qs((size_t)nq);
+        std::vector<uint64_t> qt(2 * (size_t)nq);
+        HIP_TRY(c, hipMemcpy(g.data(), c->b_gctl, sizeof(uint32_t) * mlmq::kCtlWords, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(qs.data(), c->b_qstatus, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(qt.data(), c->b_qtime, sizeof(uint64_t) * 2 * nq, hipMemcpyDeviceToHost));
+        const uint32_t derr = g[mlmq::kErrOff];
+        if (derr != mlmq::kErrNone) {
+            c->b_slots_dirty = true;
+            if (derr == mlmq::kErrOverflow && p.auto_retry && retries < 4 && cap_log2 < 31) {
+                ++retries;
+                ++cap_log2;
+                continue;
+            }
+            if (derr == mlmq::kErrOverflow) return fail(c, MLMQ_E_QUEUE_OVERFLOW, "batch queue partition overflow");
+            if (derr == mlmq::kErrTimeout) return fail(c, MLMQ_E_TIMEOUT, "device watchdog expired in batch");
+            return fail(c, MLMQ_E_INTERNAL, "batch queue accounting mismatch");
+        }
+        if (g[0] < (uint32_t)nq) return fail(c, MLMQ_E_INTERNAL, "batch ended before all queries were taken");
+        c->b_nq = nq;
+        c->b_solved = true;
+        if (timing) {
+            std::memset(timing, 0, sizeof(*timing));
+            float dev_ms = 0;
+            (void)hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_stop);
+            timing->solve_ms = ms_since(s0, s1);
+            timing->device_ms = dev_ms;
+            std::vector<double> lat((size_t)nq);
+            double sum = 0;
+            for (int32_t i = 0; i < nq; ++i) {
+                lat[i] = (double)(qt[2 * i + 1] - qt[2 * i]) / (double)c->wall_khz;
+                sum += lat[i];
+            }
+            std::sort(lat.begin(), lat.end());
+            timing->query_ms_p50 = lat[(size_t)(0.50 * (nq - 1))];
+            timing->query_ms_p99 = lat[(size_t)(0.99 * (nq - 1))];
+            timing->query_ms_max = lat.back();
+            timing->query_ms_mean = sum / nq;
+            timing->queries = nq;
+            timing->work_blocks = blocks;
+            timing->waves_per_query = p.batch_waves;
+            timing->graph_replay = use_graph ? 1 : 0;
+            timing->bucket_capacity = (int64_t)1 << cap_log2;
+            const uint64_t *st = reinterpret_cast<const uint64_t *>(g.data() + mlmq::kStatsOff);
+            timing->relaxations = st[mlmq::kStatRelax];
+            timing->improvements = st[mlmq::kStatImprove];
+            timing->l2_pushes = st[mlmq::kStatL2Push] + (uint64_t)nq;
+            timing->stale = st[mlmq::kStatStale];
+            timing->batches = st[mlmq::kStatBatches];
+            timing->idle_polls = st[mlmq::kStatIdle];
+            timing->retries = retries;
+        }
+        return MLMQ_OK;
+    }
+}
+
+int mlmq_copy_batch_dist(mlmq_ctx *c, int32_t query, int32_t *host_out) {
+    if (!c || !host_out) return MLMQ_E_INVALID;
+    if (!c->b_solved) return fail(c, MLMQ_E_NOGRAPH, "no completed batch");
+    if (query < 0 || query >= c->b_nq) return fail(c, MLMQ_E_INVALID, "query index out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpy(host_out, c->b_dist + (uint64_t)query * (uint64_t)c->n,
+                         sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToHost));
+    return MLMQ_OK;
 }
 
 int mlmq_copy_dist(mlmq_ctx *c, int32_t *host_out) {

@@ -35,6 +35,22 @@ struct SolverLaunch {
     bool profile;   // per-phase clock build of the kernel
 };
 
+struct BatchLaunch {
+    uint32_t *ctl_all;
+    uint64_t *slots_all;
+    uint32_t *dist_all;
+    const uint32_t *sources;
+    uint32_t nq;
+    uint32_t *gctl;
+    uint32_t *qstatus;
+    uint64_t *qtime;
+    int waves;   // waves per workgroup (= per query): 4, 8 or 16
+};
+
+hipError_t batch_occupancy(int waves, int *blocks_per_cu);
+hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
+                              uint32_t *gctl, hipStream_t s);
+hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s);
 hipError_t launch_reset(uint32_t *dist, uint32_t n, uint32_t src, uint32_t *ctl, uint64_t *slots,
                         hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t count, uint64_t value, hipStream_t s);

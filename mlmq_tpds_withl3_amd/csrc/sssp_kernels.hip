@@ -207,31 +207,16 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     return ok;
 }
 
-template <int kWaves, bool kSLF, bool kProf>
-__global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
-    __shared__ uint64_t s_ring[kWaves][kL1Cap];
-    __shared__ uint32_t s_map[kWaves][kWave];
-    __shared__ uint64_t s_stage[kWaves][kWave];
-
+// The worker loop of one wave for one query (reference work_block_kernel
+// loop, SS/SSSP/sssp_run.cu:507-584).  Returns when the query's exit flag is
+// seen, after an error, or at the deadline.  `leader` waves also run the
+// termination check while idle.
+template <bool kSLF, bool kProf>
+__device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
+                           uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
+                           uint64_t deadline) {
     const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x / kWave;
-    const uint32_t gwave = blockIdx.x * kWaves + w;
-    uint64_t *ring = s_ring[w];
-    uint32_t *map = s_map[w];
-    uint64_t *stage = s_stage[w];
     const L2Queue &q = a.q;
-
-    WaveStats st;
-#pragma unroll
-    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
-    PhaseClock<kProf> clk;
-    const uint64_t t_begin = kProf ? clock64() : 0;
-    clk.start();
-    if (kProf) {
-        for (int i = 0; i < 16; ++i) clk.lap(st, kClkLaps);
-    }
-
-    const uint64_t deadline = wall_clock64() + a.timeout_ticks;
     L1State l1{0u, 0u};
     uint32_t credit = 0;        // records taken from L2 whose work is not finished
     uint32_t cl_start = 0;      // lane b: start of the claim on bucket b
@@ -269,7 +254,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
                 idle_listed = true;
             }
             st.v[kStatIdle] += 1;
-            if (gwave == 0 && l2_check_done(q)) break;
+            if (leader && l2_check_done(q)) break;
             if (uniform(ld_relaxed(q.ctl + kExitOff))) break;
             if (wall_clock64() > deadline) {
                 raise_error(q, kErrTimeout);
@@ -311,18 +296,138 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
             }
         }
     }
-    if (kProf) st.v[kClkTotal] = clock64() - t_begin;
-    st.v[kStatAtomic] = st.v[kStatRelax];
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) sunk ^= pf_prev[k];
     st.v[kStatSpare2] += popc(ballot(sunk == 0x5EEDF00Du));
+}
 
-    if (lane == 0) {
-        uint64_t *stats = reinterpret_cast<uint64_t *>(q.ctl + kStatsOff);
+__device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
+    st.v[kStatAtomic] = st.v[kStatRelax];
+    if (lane_id() == 0) {
+        uint64_t *stats = reinterpret_cast<uint64_t *>(ctl + kStatsOff);
 #pragma unroll
         for (uint32_t i = 0; i < kNumStats; ++i)
             if (st.v[i]) add64(stats + i, st.v[i]);
     }
+}
+
+template <int kWaves, bool kSLF, bool kProf>
+__global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
+    __shared__ uint64_t s_ring[kWaves][kL1Cap];
+    __shared__ uint32_t s_map[kWaves][kWave];
+    __shared__ uint64_t s_stage[kWaves][kWave];
+
+    const uint32_t w = threadIdx.x / kWave;
+    const uint32_t gwave = blockIdx.x * kWaves + w;
+
+    WaveStats st;
+#pragma unroll
+    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    PhaseClock<kProf> clk;
+    const uint64_t t_begin = kProf ? clock64() : 0;
+    clk.start();
+    if (kProf) {
+        for (int i = 0; i < 16; ++i) clk.lap(st, kClkLaps);
+    }
+    const uint64_t deadline = wall_clock64() + a.timeout_ticks;
+    run_worker<kSLF, kProf>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline);
+    if (kProf) st.v[kClkTotal] = clock64() - t_begin;
+    flush_stats(a.q.ctl, st);
+}
+
+// ---- many queries in one persistent launch ----
+//
+// Every workgroup owns a private L2 partition (control block + bucket rings)
+// and works through queries taken from a global counter, one at a time, with
+// all of its waves.  Between queries the workgroup resets its control block
+// and seeds the next source behind a workgroup barrier; its rings are empty
+// after every completed query (each reserved slot was taken), and the
+// distance rows were filled for all queries before the launch.  The query a
+// record belongs to is the one its partition is serving, so records stay
+// 8 bytes.
+struct BatchArgs {
+    SolveArgs base;            // graph, queue parameters, timeout
+    uint32_t *ctl_all;         // grid x kCtlWords
+    uint64_t *slots_all;       // grid x (nbuckets << cap_log2)
+    uint32_t *dist_all;        // nq x n
+    const uint32_t *sources;   // nq
+    uint32_t nq;
+    uint32_t *gctl;            // launch-wide: [0] next query, kErrOff error, kStatsOff stats
+    uint32_t *qstatus;         // per query: device error code (0 = ok)
+    uint64_t *qtime;           // per query: wall-clock ticks at start and end
+};
+
+template <int kWaves, bool kProf>
+__global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
+    __shared__ uint64_t s_ring[kWaves][kL1Cap];
+    __shared__ uint32_t s_map[kWaves][kWave];
+    __shared__ uint64_t s_stage[kWaves][kWave];
+    __shared__ uint32_t s_query;
+
+    const uint32_t w = threadIdx.x / kWave;
+    SolveArgs a = b.base;
+    a.q.ctl = b.ctl_all + (uint64_t)blockIdx.x * kCtlWords;
+    a.q.slots = b.slots_all + ((uint64_t)blockIdx.x * a.q.nbuckets << a.q.cap_log2);
+
+    WaveStats st;
+#pragma unroll
+    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    PhaseClock<kProf> clk;
+    clk.start();
+    const uint64_t deadline = wall_clock64() + a.timeout_ticks;
+
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t qi = fetch_add(b.gctl, 1u);
+            if (ld_relaxed(b.gctl + kErrOff) != 0) qi = b.nq;
+            s_query = qi;
+        }
+        __syncthreads();
+        const uint32_t qi = s_query;
+        if (qi >= b.nq) break;
+        for (uint32_t i = threadIdx.x; i < kStatsOff; i += blockDim.x)
+            a.q.ctl[i] = i == kTailOff ? 1u : 0u;
+        if (threadIdx.x == 0) {
+            a.q.slots[0] = make_rec(b.sources[qi], 0u);
+            b.qtime[2 * qi] = wall_clock64();
+        }
+        __threadfence();
+        __syncthreads();
+        a.dist = b.dist_all + (uint64_t)qi * a.n;
+        run_worker<true, kProf>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            b.qtime[2 * qi + 1] = wall_clock64();
+            uint32_t e = ld_relaxed(a.q.ctl + kErrOff);
+            if (e == kErrNone) {
+                // the partition's accounting must balance, as after a single solve
+                uint32_t t = 0, k = 0;
+                for (uint32_t i = 0; i < a.q.nbuckets; ++i) {
+                    t += ld_relaxed(a.q.ctl + kTailOff + i * kStride);
+                    k += ld_relaxed(a.q.ctl + kTakenOff + i * kStride);
+                }
+                if (t != k || t != ld_relaxed(a.q.ctl + kDoneOff)) e = kErrInternal;
+            }
+            b.qstatus[qi] = e;
+            if (e != kErrNone) cas32(b.gctl + kErrOff, kErrNone, e);
+        }
+        __syncthreads();
+    }
+    flush_stats(b.gctl, st);
+}
+
+// dist rows of all queries (INT_MAX, 0 at the source), the query counter and
+// the launch-wide error/stats words.
+__global__ void batch_reset_kernel(uint32_t *dist_all, uint32_t n, uint32_t nq,
+                                   const uint32_t *sources, uint32_t *gctl) {
+    const uint64_t total = (uint64_t)n * nq;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t qi = (uint32_t)(i / n), v = (uint32_t)(i - (uint64_t)qi * n);
+        dist_all[i] = v == sources[qi] ? 0u : kUnreached;
+    }
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < kCtlWords) gctl[tid] = 0u;
 }
 
 // Hop blocks from the CSR (layout in hop_block.h).
@@ -432,6 +537,64 @@ static const void *solver_fn(bool slf) {
 hipError_t solver_occupancy(bool slf, int *blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, solver_fn(slf),
                                                         kWavesPerBlock * kWave, 0);
+}
+
+template <int kW>
+static const void *batch_fn() {
+    return reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false>);
+}
+
+hipError_t batch_occupancy(int waves, int *blocks_per_cu) {
+    const void *fn = waves == 16 ? batch_fn<16>() : waves == 8 ? batch_fn<8>() : batch_fn<4>();
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
+}
+
+hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
+                              uint32_t *gctl, hipStream_t s) {
+    const uint64_t work = (uint64_t)n * nq > kCtlWords ? (uint64_t)n * nq : kCtlWords;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, dist_all, n,
+                       nq, sources, gctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s) {
+    BatchArgs b;
+    b.base.row = L.row;
+    b.base.col = L.col;
+    b.base.wgt = L.wgt;
+    b.base.blocks = L.blocks;
+    b.base.dist = nullptr;
+    b.base.q.slots = nullptr;
+    b.base.q.ctl = nullptr;
+    b.base.q.nbuckets = L.nbuckets;
+    b.base.q.window = L.window;
+    b.base.q.cap_log2 = L.cap_log2;
+    b.base.q.delta = L.delta;
+    b.base.q.batch = L.batch;
+    b.base.q.spin_limit = L.spin_limit;
+    b.base.n = L.n;
+    b.base.min_read = L.min_read;
+    b.base.max_sleep = L.max_sleep;
+    b.base.pop_max = L.pop_max;
+    b.base.spill = L.spill;
+    b.base.donate_min = L.donate_min;
+    b.base.timeout_ticks = L.timeout_ticks;
+    b.ctl_all = B.ctl_all;
+    b.slots_all = B.slots_all;
+    b.dist_all = B.dist_all;
+    b.sources = B.sources;
+    b.nq = B.nq;
+    b.gctl = B.gctl;
+    b.qstatus = B.qstatus;
+    b.qtime = B.qtime;
+    const dim3 grid(L.grid_blocks);
+    if (B.waves == 16)
+        hipLaunchKernelGGL((sssp_batch_kernel<16, false>), grid, dim3(16 * kWave), 0, s, b);
+    else if (B.waves == 8)
+        hipLaunchKernelGGL((sssp_batch_kernel<8, false>), grid, dim3(8 * kWave), 0, s, b);
+    else
+        hipLaunchKernelGGL((sssp_batch_kernel<4, false>), grid, dim3(4 * kWave), 0, s, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {

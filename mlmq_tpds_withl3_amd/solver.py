@@ -19,7 +19,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import GraphInfo, MLMQError, Params, Timing, check  # noqa: F401
+from ._lib import BatchTiming, GraphInfo, MLMQError, Params, Timing, check  # noqa: F401
 
 UNREACHED = 0x7FFFFFFF
 
@@ -119,6 +119,21 @@ class Solver:
         t = Timing()
         check(self._L.mlmq_solve(self._h, int(src), ctypes.byref(p), ctypes.byref(t)), self._h)
         return t.as_dict()
+
+    def solve_batch(self, sources, params: Params | None = None, **overrides) -> dict:
+        """All `sources` in one persistent launch; distances stay on the device."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        p = params if params is not None else default_params(**overrides)
+        t = BatchTiming()
+        check(self._L.mlmq_solve_batch(self._h, src.ctypes.data, len(src), ctypes.byref(p),
+                                       ctypes.byref(t)), self._h)
+        self._batch_n = len(src)
+        return t.as_dict()
+
+    def batch_dist(self, query: int) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        check(self._L.mlmq_copy_batch_dist(self._h, int(query), out.ctypes.data), self._h)
+        return out
 
     def dist(self) -> np.ndarray:
         out = np.empty(self.n, dtype=np.int32)

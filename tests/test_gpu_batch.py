@@ -1,0 +1,113 @@
+"""Bit-exact parity of the batched (many queries, one persistent launch) solver.
+
+Every query of every batch must equal the CPU oracle.  Cases: the golden
+fixture sources, partitions reused across many queries (fewer workgroups than
+queries), every supported workgroup size, hipGraph replays interleaved with
+eager launches and single solves, a road-like graph, and error paths.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from graphs_util import random_csr, grid_csr
+from mlmq_tpds_withl3_amd import Solver, MLMQError, make_workload
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TIMEOUT = 20.0
+
+
+def say(msg):
+    print(f"[gpu-batch] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = Solver(0)
+    yield s
+    s.close()
+
+
+def check_batch(solver, row, col, w, sources, **params):
+    params.setdefault("timeout_s", TIMEOUT)
+    t = solver.solve_batch(sources, **params)
+    cache = {}
+    for i, src in enumerate(sources):
+        if src not in cache:
+            cache[src] = O.dijkstra(row, col, w, int(src))[0]
+        got = solver.batch_dist(i)
+        bad = np.flatnonzero(got != cache[src])
+        assert bad.size == 0, f"query {i} (src {src}): {bad.size} mismatches {params}"
+    return t
+
+
+def test_fixture_golden_batch(solver):
+    say("fixture golden batch")
+    solver.load_gr(os.path.join(GOLDEN, "fixture.gr"))
+    sources = [0, 1030, 2050, 0, 1030, 2050, 0]
+    t = solver.solve_batch(sources, timeout_s=TIMEOUT, work_blocks=2)
+    for i, src in enumerate(sources):
+        exp = np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), dtype=np.int32)
+        assert np.array_equal(solver.batch_dist(i), exp), f"query {i}"
+    assert t["queries"] == len(sources) and t["work_blocks"] == 2
+
+
+@pytest.mark.parametrize("waves", [4, 8, 16])
+@pytest.mark.parametrize("blocks", [1, 5, 0])
+def test_random_batch(solver, waves, blocks):
+    say(f"random batch waves={waves} blocks={blocks}")
+    row, col, w = random_csr(3000, 12000, 17, wmax=500, zero_frac=0.1)
+    solver.load_csr(row, col, w)
+    rng = np.random.default_rng(waves * 10 + blocks)
+    sources = rng.integers(0, 3000, size=40)
+    check_batch(solver, row, col, w, sources, batch_waves=waves, work_blocks=blocks, delta=200)
+
+
+def test_graph_replays_interleaved(solver):
+    say("graph replays with eager work in between")
+    row, col, w = grid_csr(80, 90, 21)
+    solver.load_csr(row, col, w)
+    n = len(row) - 1
+    rng = np.random.default_rng(3)
+    sources = rng.integers(0, n, size=24)
+    eager = check_batch(solver, row, col, w, sources, use_graph=0, work_blocks=6)
+    assert eager["graph_replay"] == 0
+    for rep in range(3):
+        t = check_batch(solver, row, col, w, sources, use_graph=1, work_blocks=6)
+        assert t["graph_replay"] == 1
+        # the program's own eager work and a different query between replays
+        solver.solve(int(sources[rep]), timeout_s=TIMEOUT)
+        ref, _, _ = O.dijkstra(row, col, w, int(sources[rep]))
+        assert np.array_equal(solver.dist(), ref)
+    # a new source list reuses the captured graph (sources are copied in first)
+    check_batch(solver, row, col, w, rng.integers(0, n, size=24), use_graph=1, work_blocks=6)
+
+
+def test_road264k_batch(solver):
+    say("road264k batch of 48")
+    g, _, _ = make_workload("road264k")
+    row, col, w = g.arrays()
+    solver.load_csr(row, col, w)
+    sources = np.random.default_rng(7).integers(0, g.n, size=48)
+    t = check_batch(solver, row, col, w, sources)
+    say(f"road264k batch solve_ms={t['solve_ms']:.2f} p99={t['query_ms_p99']:.2f}")
+
+
+def test_batch_errors(solver):
+    say("batch errors")
+    row, col, w = grid_csr(100, 100, 5)
+    solver.load_csr(row, col, w)
+    with pytest.raises(MLMQError) as e:
+        solver.solve_batch([0, 10_000])
+    assert e.value.code == -1
+    with pytest.raises(MLMQError) as e:
+        solver.solve_batch([0, 1], batch_waves=5)
+    assert e.value.code == -1
+    with pytest.raises(MLMQError) as e:
+        solver.solve_batch(list(range(16)), timeout_s=1e-7, delta=10)
+    assert e.value.code == -7
+    check_batch(solver, row, col, w, list(range(0, 10_000, 700)))
