@@ -79,6 +79,10 @@ __device__ __forceinline__ bool push_one(uint64_t *ring, L1State &l1, uint64_t r
 // Heavy records (more out-edges than a hop block holds): edge slots spread
 // over the lanes by a DPP scan of degrees, the owner of a slot found as a DPP
 // running maximum of start markers, columns and weights read from the CSR.
+// kHeavyUnroll chunks of 64 slots are handled per iteration: all their
+// column/weight loads, then all their atomicMins, are in flight together.
+constexpr int kHeavyUnroll = 4;
+
 template <bool kSLF>
 __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
                             uint32_t hdeg, uint64_t *ring, L1State &l1, uint32_t *map,
@@ -87,32 +91,71 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
     const uint32_t deg = heavy ? hdeg : 0u;
     uint32_t total = 0;
     const uint32_t excl = wave_excl_scan(deg, total);
+    // per lane: offset of its record's first edge relative to its first slot
+    const uint32_t off = hstart - excl;
     uint32_t carry = 0;
     bool ok = true;
-    for (uint32_t base = 0; base < total && ok; base += kWave) {
-        const uint32_t qi = base + lane;
-        const bool act = qi < total;
-        map[lane] = 0;
-        wave_sync();
-        if (deg > 0 && excl >= base && excl - base < (uint32_t)kWave) map[excl - base] = lane + 1;
-        wave_sync();
-        const uint32_t o1 = max(carry, wave_incl_max(map[lane]));
-        carry = read_lane(o1, kWave - 1);
-        const uint32_t o = act ? o1 - 1 : 0u;
-        const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)o, kWave);
-        const uint32_t o_start = (uint32_t)__shfl((int)hstart, (int)o, kWave);
-        const uint32_t o_d = (uint32_t)__shfl((int)d, (int)o, kWave);
-        bool improved = false;
-        uint32_t c = 0, nd = 0;
-        if (act) {
-            const uint32_t e = o_start + (qi - o_excl);
-            c = a.col[e];
-            nd = o_d + a.wgt[e];
-            improved = nd < fetch_min(a.dist + c, nd);
+    for (uint32_t base = 0; base < total && ok; base += kHeavyUnroll * kWave) {
+        uint32_t o[kHeavyUnroll];
+        bool act[kHeavyUnroll];
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) {
+            const uint32_t cb = base + j * kWave;
+            act[j] = cb + lane < total;
+            map[lane] = 0;
+            wave_sync();
+            if (deg > 0 && excl >= cb && excl - cb < (uint32_t)kWave) map[excl - cb] = lane + 1;
+            wave_sync();
+            const uint32_t o1 = max(carry, wave_incl_max(map[lane]));
+            carry = read_lane(o1, kWave - 1);
+            o[j] = act[j] ? o1 - 1 : 0u;
         }
-        st.v[kStatRelax] += popc(ballot(act));
-        st.v[kStatImprove] += popc(ballot(improved));
-        ok = push_one<kSLF>(ring, l1, make_rec(c, nd), improved, a, st);
+        uint32_t c[kHeavyUnroll], nd[kHeavyUnroll], o_d[kHeavyUnroll];
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) {
+            const uint32_t o_off = (uint32_t)__shfl((int)off, (int)o[j], kWave);
+            o_d[j] = (uint32_t)__shfl((int)d, (int)o[j], kWave);
+            c[j] = 0;
+            nd[j] = 0;
+            if (act[j]) {
+                const uint32_t e = o_off + base + j * kWave + lane;
+                c[j] = a.col[e];
+                nd[j] = a.wgt[e];
+            }
+        }
+        // all column/weight loads issued before any is used
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
+        uint32_t pv[kHeavyUnroll];
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) {
+            pv[j] = 0;
+            if (act[j]) pv[j] = fetch_min(a.dist + c[j], nd[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pv[j]));
+        uint64_t out[kHeavyUnroll], im[kHeavyUnroll];
+        uint32_t relax = 0, improved = 0;
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) {
+            im[j] = ballot(act[j] && nd[j] < pv[j]);
+            out[j] = make_rec(c[j], nd[j]);
+            relax += popc(ballot(act[j]));
+            improved += popc(im[j]);
+        }
+        st.v[kStatRelax] += relax;
+        st.v[kStatImprove] += improved;
+        if (improved) {
+            if (kSLF) {
+                ok = l1_push<kHeavyUnroll>(ring, l1, out, im, a.spill, a.q, st);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kHeavyUnroll; ++j)
+                    if (ok && im[j]) ok = l2_push(a.q, out[j], (im[j] >> lane) & 1, st);
+            }
+        }
     }
     return ok;
 }
