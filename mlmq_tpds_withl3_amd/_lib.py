@@ -1,4 +1,5 @@
-"""ctypes binding of ``libmlmq.so`` (include/mlmq.h, include/mlmq_graph.h).
+"""ctypes binding of ``libmlmq.so`` (include/mlmq.h, include/mlmq_graph.h,
+include/mlmq_serve.h).
 
 The library is built in-tree by ``make -C mlmq_tpds_withl3_amd/csrc``; this
 module never builds or falls back to anything: if the shared object is missing
@@ -17,7 +18,7 @@ MLMQ_OK = 0
 ERRORS = {
     -1: "MLMQ_E_INVALID", -2: "MLMQ_E_NOGRAPH", -3: "MLMQ_E_HIP", -4: "MLMQ_E_NOMEM",
     -5: "MLMQ_E_IO", -6: "MLMQ_E_QUEUE_OVERFLOW", -7: "MLMQ_E_TIMEOUT",
-    -8: "MLMQ_E_INTERNAL", -9: "MLMQ_E_UNSUPPORTED",
+    -8: "MLMQ_E_INTERNAL", -9: "MLMQ_E_UNSUPPORTED", -10: "MLMQ_E_REJECTED",
 }
 QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1}
 
@@ -33,6 +34,12 @@ GRAPH_SYMBOLS = [
     "mlmq_graph_from_csr", "mlmq_graph_write_gr", "mlmq_graph_dims", "mlmq_graph_arrays",
     "mlmq_graph_coords", "mlmq_graph_weight_scale", "mlmq_graph_centroid_vertex", "mlmq_graph_max_degree_vertex",
     "mlmq_graph_free",
+]
+SERVE_SYMBOLS = [
+    "mlmq_server_config_default", "mlmq_serve_graph_params_default", "mlmq_server_create",
+    "mlmq_server_destroy", "mlmq_server_last_error", "mlmq_server_add_csr", "mlmq_server_add_gr",
+    "mlmq_server_set_graph_params", "mlmq_server_graph_size", "mlmq_server_slot_bytes",
+    "mlmq_server_set_expected", "mlmq_server_clear_expected", "mlmq_server_run",
 ]
 
 
@@ -108,6 +115,62 @@ class GraphInfo(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_uint64), ("streams", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32), ("max_pending", ctypes.c_int32),
+        ("verify", ctypes.c_int32), ("timeout_s", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class ServeGraphParams(ctypes.Structure):
+    _fields_ = [
+        ("delta", ctypes.c_uint32), ("buckets", ctypes.c_int32),
+        ("bucket_window", ctypes.c_int32), ("l2_batch", ctypes.c_int32),
+        ("min_read", ctypes.c_int32), ("l1_pop", ctypes.c_int32),
+        ("l1_spill", ctypes.c_int32), ("workgroups", ctypes.c_int32),
+        ("waves", ctypes.c_int32), ("bucket_capacity", ctypes.c_int64),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double), ("graph", ctypes.c_int32), ("src", ctypes.c_int32),
+        ("dist_out", ctypes.c_void_p),
+    ]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [
+        ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double), ("status", ctypes.c_int32),
+        ("mismatches", ctypes.c_int32), ("launch", ctypes.c_int32),
+        ("batch_size", ctypes.c_int32), ("retries", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("wall_s", ctypes.c_double), ("offered_qps", ctypes.c_double),
+        ("completed_qps", ctypes.c_double), ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+        ("queue_p50_ms", ctypes.c_double), ("queue_p99_ms", ctypes.c_double),
+        ("mean_batch", ctypes.c_double), ("mean_device_ms", ctypes.c_double),
+        ("requests", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+        ("verified", ctypes.c_uint64), ("mismatched", ctypes.c_uint64),
+        ("retries", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+        ("peak_bytes", ctypes.c_uint64), ("peak_inflight", ctypes.c_uint64),
+        ("reserved", ctypes.c_int64 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 _lib = None
 
 
@@ -162,6 +225,24 @@ def load() -> ctypes.CDLL:
         "mlmq_graph_centroid_vertex": (i32, [P]),
         "mlmq_graph_max_degree_vertex": (i32, [P]),
         "mlmq_graph_free": (None, [P]),
+        "mlmq_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
+        "mlmq_serve_graph_params_default": (None, [ctypes.POINTER(ServeGraphParams)]),
+        "mlmq_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerConfig),
+                                              ctypes.POINTER(P)]),
+        "mlmq_server_destroy": (ctypes.c_int, [P]),
+        "mlmq_server_last_error": (ctypes.c_char_p, [P]),
+        "mlmq_server_add_csr": (ctypes.c_int, [P, i32, i64, P, P, P,
+                                               ctypes.POINTER(ServeGraphParams),
+                                               ctypes.POINTER(i32)]),
+        "mlmq_server_add_gr": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.POINTER(ServeGraphParams),
+                                              ctypes.POINTER(i32)]),
+        "mlmq_server_set_graph_params": (ctypes.c_int, [P, i32, ctypes.POINTER(ServeGraphParams)]),
+        "mlmq_server_graph_size": (ctypes.c_int, [P, i32, ctypes.POINTER(i32), ctypes.POINTER(i64)]),
+        "mlmq_server_slot_bytes": (ctypes.c_int, [P, i32, ctypes.POINTER(u64)]),
+        "mlmq_server_set_expected": (ctypes.c_int, [P, i32, i32, P]),
+        "mlmq_server_clear_expected": (ctypes.c_int, [P]),
+        "mlmq_server_run": (ctypes.c_int, [P, ctypes.POINTER(ServeRequest), i32,
+                                           ctypes.POINTER(ServeResult), ctypes.POINTER(ServeStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -252,6 +252,21 @@ int ensure_queue(mlmq_ctx *c, uint32_t nbuckets, uint32_t cap_log2) {
 
 }  // namespace
 
+int mlmq::ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v) {
+    if (!c || !v) return MLMQ_E_INVALID;
+    if (!c->row) return MLMQ_E_NOGRAPH;
+    v->row = c->row;
+    v->col = c->col;
+    v->wgt = c->wgt;
+    v->blocks = c->blocks;
+    v->n = c->n;
+    v->m = c->m;
+    v->device = c->device;
+    v->cus = c->cus;
+    v->wall_khz = c->wall_khz;
+    return MLMQ_OK;
+}
+
 extern "C" {
 
 const char *mlmq_version(void) { return "mlmq-gfx950 0.1"; }

@@ -5,7 +5,24 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct mlmq_ctx;
+
 namespace mlmq {
+
+// Device graph held by a context (used by the serving loop, which keeps one
+// context per resident graph).
+struct DeviceGraphView {
+    const uint32_t *row;
+    const uint32_t *col;
+    const uint32_t *wgt;
+    const uint4 *blocks;
+    int32_t n;
+    int64_t m;
+    int device;
+    int cus;
+    int wall_khz;
+};
+int ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v);
 
 constexpr int kWavesPerBlock = 4;   // 256-thread persistent workgroups
 
@@ -47,6 +64,32 @@ struct BatchLaunch {
     int waves;   // waves per workgroup (= per query): 4, 8 or 16
 };
 
+// Host mirror of the device-side ServeRequest (sssp_kernels.hip); the layout
+// must match field for field.
+struct ServeRequestHost {
+    uint32_t src;
+    uint32_t status;
+    uint32_t fresh;
+    uint32_t mismatches;
+    uint32_t *dist;
+    uint32_t *ctl;
+    uint64_t *slots;
+    const uint32_t *expect;
+    uint64_t t_start;
+    uint64_t t_end;
+};
+static_assert(sizeof(ServeRequestHost) == 64, "request layout");
+
+struct ServeLaunch {
+    ServeRequestHost *reqs;   // device table, nreq entries
+    uint32_t nreq;
+    uint32_t wg_per_req;      // workgroups sharing one request's partition
+    int waves;                // waves per workgroup: 4, 8 or 16
+};
+
+hipError_t serve_occupancy(int waves, int *blocks_per_cu);
+hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t s);
+hipError_t launch_serve_verify(const ServeLaunch &S, uint32_t n, hipStream_t s);
 hipError_t batch_occupancy(int waves, int *blocks_per_cu);
 hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
                               uint32_t *gctl, hipStream_t s);

@@ -459,6 +459,101 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
     flush_stats(b.gctl, st);
 }
 
+// ---- serving: requests of one resident graph, several workgroups each ----
+//
+// A serving launch carries a table of requests for one graph.  Each request
+// owns a slot handed out by the host's admission control: a distance row and
+// a queue partition (control block + buckets).  serve_reset_kernel prepares
+// the slots (row fill, control reset, empty-slot fill of fresh partitions,
+// seed record); sssp_serve_kernel gives every request `wg_per_req`
+// workgroups that share its partition; serve_verify_kernel compares finished
+// rows with expected rows when the caller registered them.
+struct ServeRequest {
+    uint32_t src;
+    uint32_t status;          // out: device error code
+    uint32_t fresh;           // partition content unknown: fill with empty slots
+    uint32_t mismatches;      // out (verification)
+    uint32_t *dist;
+    uint32_t *ctl;
+    uint64_t *slots;
+    const uint32_t *expect;   // optional expected row
+    uint64_t t_start;         // out: wall-clock ticks of the first workgroup's start
+    uint64_t t_end;           // out: latest workgroup end
+};
+
+// grid (x: row chunks, y: requests)
+__global__ __launch_bounds__(256) void serve_reset_kernel(ServeRequest *reqs, uint32_t n,
+                                                          uint64_t part_slots) {
+    const ServeRequest r = reqs[blockIdx.y];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t0; i < n; i += stride) r.dist[i] = i == r.src ? 0u : kUnreached;
+    if (r.fresh)
+        for (uint64_t i = t0 + 1; i < part_slots; i += stride) r.slots[i] = kEmpty;
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < kStatsOff; i += blockDim.x) r.ctl[i] = i == kTailOff ? 1u : 0u;
+        if (threadIdx.x == 0) r.slots[0] = make_rec(r.src, 0u);
+    }
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base, ServeRequest *reqs,
+                                                                 uint32_t wg_per_req) {
+    __shared__ uint64_t s_ring[kWaves][kL1Cap];
+    __shared__ uint32_t s_map[kWaves][kWave];
+    __shared__ uint64_t s_stage[kWaves][kWave];
+
+    const uint32_t w = threadIdx.x / kWave;
+    const uint32_t ri = blockIdx.x / wg_per_req, part = blockIdx.x % wg_per_req;
+    ServeRequest *r = reqs + ri;
+    SolveArgs a = base;
+    a.dist = r->dist;
+    a.q.ctl = r->ctl;
+    a.q.slots = r->slots;
+    if (part == 0 && threadIdx.x == 0) r->t_start = wall_clock64();
+
+    WaveStats st;
+#pragma unroll
+    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    PhaseClock<false> clk;
+    const uint64_t deadline = wall_clock64() + a.timeout_ticks;
+    run_worker<true, false>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (part == 0) {
+            // Every record pushed was claimed and credited: exit was raised by
+            // the quiescence check, so the control block no longer changes.
+            uint32_t e = ld_relaxed(a.q.ctl + kErrOff);
+            if (e == kErrNone) {
+                uint32_t t = 0, k = 0;
+                for (uint32_t i = 0; i < a.q.nbuckets; ++i) {
+                    t += ld_relaxed(a.q.ctl + kTailOff + i * kStride);
+                    k += ld_relaxed(a.q.ctl + kTakenOff + i * kStride);
+                }
+                if (t != k || t != ld_relaxed(a.q.ctl + kDoneOff)) e = kErrInternal;
+            }
+            r->status = e;
+        }
+        __hip_atomic_fetch_max(&r->t_end, (uint64_t)wall_clock64(), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// grid (x: row chunks, y: requests)
+__global__ __launch_bounds__(256) void serve_verify_kernel(ServeRequest *reqs, uint32_t n) {
+    ServeRequest *r = reqs + blockIdx.y;
+    const uint32_t *expect = r->expect;
+    if (!expect) return;
+    const uint32_t *dist = r->dist;
+    uint32_t bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        bad += dist[i] != expect[i] ? 1u : 0u;
+    bad = wave_sum(bad);
+    if (lane_id() == 0 && bad) atomicAdd(&r->mismatches, bad);
+}
+
 // dist rows of all queries (INT_MAX, 0 at the source), the query counter and
 // the launch-wide error/stats words.
 __global__ void batch_reset_kernel(uint32_t *dist_all, uint32_t n, uint32_t nq,
@@ -592,55 +687,8 @@ hipError_t batch_occupancy(int waves, int *blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
 }
 
-hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
-                              uint32_t *gctl, hipStream_t s) {
-    const uint64_t work = (uint64_t)n * nq > kCtlWords ? (uint64_t)n * nq : kCtlWords;
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, dist_all, n,
-                       nq, sources, gctl);
-    return hipGetLastError();
-}
-
-hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s) {
-    BatchArgs b;
-    b.base.row = L.row;
-    b.base.col = L.col;
-    b.base.wgt = L.wgt;
-    b.base.blocks = L.blocks;
-    b.base.dist = nullptr;
-    b.base.q.slots = nullptr;
-    b.base.q.ctl = nullptr;
-    b.base.q.nbuckets = L.nbuckets;
-    b.base.q.window = L.window;
-    b.base.q.cap_log2 = L.cap_log2;
-    b.base.q.delta = L.delta;
-    b.base.q.batch = L.batch;
-    b.base.q.spin_limit = L.spin_limit;
-    b.base.n = L.n;
-    b.base.min_read = L.min_read;
-    b.base.max_sleep = L.max_sleep;
-    b.base.pop_max = L.pop_max;
-    b.base.spill = L.spill;
-    b.base.donate_min = L.donate_min;
-    b.base.timeout_ticks = L.timeout_ticks;
-    b.ctl_all = B.ctl_all;
-    b.slots_all = B.slots_all;
-    b.dist_all = B.dist_all;
-    b.sources = B.sources;
-    b.nq = B.nq;
-    b.gctl = B.gctl;
-    b.qstatus = B.qstatus;
-    b.qtime = B.qtime;
-    const dim3 grid(L.grid_blocks);
-    if (B.waves == 16)
-        hipLaunchKernelGGL((sssp_batch_kernel<16, false>), grid, dim3(16 * kWave), 0, s, b);
-    else if (B.waves == 8)
-        hipLaunchKernelGGL((sssp_batch_kernel<8, false>), grid, dim3(8 * kWave), 0, s, b);
-    else
-        hipLaunchKernelGGL((sssp_batch_kernel<4, false>), grid, dim3(4 * kWave), 0, s, b);
-    return hipGetLastError();
-}
-
-hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
+// Kernel arguments from a host launch description.
+static SolveArgs make_args(const SolverLaunch &L) {
     SolveArgs a;
     a.row = L.row;
     a.col = L.col;
@@ -662,6 +710,75 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     a.spill = L.spill;
     a.donate_min = L.donate_min;
     a.timeout_ticks = L.timeout_ticks;
+    return a;
+}
+
+hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
+                              uint32_t *gctl, hipStream_t s) {
+    const uint64_t work = (uint64_t)n * nq > kCtlWords ? (uint64_t)n * nq : kCtlWords;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, dist_all, n,
+                       nq, sources, gctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s) {
+    BatchArgs b;
+    b.base = make_args(L);
+    b.ctl_all = B.ctl_all;
+    b.slots_all = B.slots_all;
+    b.dist_all = B.dist_all;
+    b.sources = B.sources;
+    b.nq = B.nq;
+    b.gctl = B.gctl;
+    b.qstatus = B.qstatus;
+    b.qtime = B.qtime;
+    const dim3 grid(L.grid_blocks);
+    if (B.waves == 16)
+        hipLaunchKernelGGL((sssp_batch_kernel<16, false>), grid, dim3(16 * kWave), 0, s, b);
+    else if (B.waves == 8)
+        hipLaunchKernelGGL((sssp_batch_kernel<8, false>), grid, dim3(8 * kWave), 0, s, b);
+    else
+        hipLaunchKernelGGL((sssp_batch_kernel<4, false>), grid, dim3(4 * kWave), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t serve_occupancy(int waves, int *blocks_per_cu) {
+    const void *fn = waves == 16 ? reinterpret_cast<const void *>(&sssp_serve_kernel<16>)
+                     : waves == 8 ? reinterpret_cast<const void *>(&sssp_serve_kernel<8>)
+                                  : reinterpret_cast<const void *>(&sssp_serve_kernel<4>);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
+}
+
+hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t s) {
+    if (S.nreq == 0) return hipSuccess;
+    SolveArgs a = make_args(L);
+    auto *reqs = reinterpret_cast<ServeRequest *>(S.reqs);
+    const uint64_t part_slots = (uint64_t)L.nbuckets << L.cap_log2;
+    const uint64_t longest = part_slots > L.n ? part_slots : (uint64_t)L.n;
+    uint32_t chunks = (uint32_t)((longest + 256 * 64 - 1) / (256 * 64));
+    if (chunks > 512) chunks = 512;
+    hipLaunchKernelGGL(serve_reset_kernel, dim3(chunks, S.nreq), dim3(256), 0, s, reqs, L.n, part_slots);
+    const dim3 grid(S.nreq * S.wg_per_req);
+    if (S.waves == 16)
+        hipLaunchKernelGGL((sssp_serve_kernel<16>), grid, dim3(16 * kWave), 0, s, a, reqs, S.wg_per_req);
+    else if (S.waves == 8)
+        hipLaunchKernelGGL((sssp_serve_kernel<8>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
+    else
+        hipLaunchKernelGGL((sssp_serve_kernel<4>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_verify(const ServeLaunch &S, uint32_t n, hipStream_t s) {
+    if (S.nreq == 0) return hipSuccess;
+    uint32_t chunks = (uint32_t)(((uint64_t)n + 256 * 16 - 1) / (256 * 16));
+    if (chunks > 256) chunks = 256;
+    hipLaunchKernelGGL(serve_verify_kernel, dim3(chunks, S.nreq), dim3(256), 0, s,
+                       reinterpret_cast<ServeRequest *>(S.reqs), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
+    const SolveArgs a = make_args(L);
     const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
     if (L.slf && !L.profile)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false>), grid, block, 0, s, a);

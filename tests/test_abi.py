@@ -10,7 +10,7 @@ import pytest
 from mlmq_tpds_withl3_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("mlmq.h", "mlmq_graph.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("mlmq.h", "mlmq_graph.h", "mlmq_serve.h")]
 
 
 def declared_functions(path):
@@ -21,8 +21,9 @@ def declared_functions(path):
 
 
 def test_headers_parse():
-    names = declared_functions(HEADERS[0]) + declared_functions(HEADERS[1])
+    names = sum((declared_functions(h) for h in HEADERS), [])
     assert "mlmq_solve" in names and "sssp_init" in names and "mlmq_graph_read_gr" in names
+    assert "mlmq_server_run" in names
 
 
 @pytest.mark.parametrize("header", HEADERS)
@@ -33,8 +34,8 @@ def test_every_declared_symbol_is_exported(header):
 
 
 def test_binding_lists_match_headers():
-    declared = set(declared_functions(HEADERS[0])) | set(declared_functions(HEADERS[1]))
-    assert declared == set(_lib.SOLVER_SYMBOLS) | set(_lib.GRAPH_SYMBOLS)
+    declared = set().union(*(declared_functions(h) for h in HEADERS))
+    assert declared == set(_lib.SOLVER_SYMBOLS) | set(_lib.GRAPH_SYMBOLS) | set(_lib.SERVE_SYMBOLS)
 
 
 def test_loader_binds_all():
@@ -61,3 +62,46 @@ def test_cli_built():
     assert os.access(_lib.CLI_PATH, os.X_OK)
     r = subprocess.run([_lib.CLI_PATH], capture_output=True, text=True)
     assert r.returncode == 2 and "Usage" in r.stdout
+
+
+def test_serving_defaults():
+    L = _lib.load()
+    c = _lib.ServerConfig()
+    L.mlmq_server_config_default(ctypes.byref(c))
+    assert (c.hbm_budget, c.streams, c.max_batch, c.max_pending, c.verify) == (32 << 30, 4, 64, 0, 1)
+    g = _lib.ServeGraphParams()
+    L.mlmq_serve_graph_params_default(ctypes.byref(g))
+    assert (g.delta, g.buckets, g.bucket_window, g.l2_batch, g.min_read) == (200000, 16, 4, 8, 8)
+    assert (g.l1_pop, g.l1_spill, g.workgroups, g.waves, g.bucket_capacity) == (32, 64, 0, 8, 0)
+
+
+def test_ctypes_layouts_match_c(tmp_path):
+    """Every structure the binding mirrors has the size and field offsets the
+    C compiler gives it."""
+    structs = {
+        "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph"]),
+        "mlmq_timing": (_lib.Timing, None),
+        "mlmq_batch_timing": (_lib.BatchTiming, None),
+        "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
+        "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "reserved"]),
+        "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "reserved"]),
+        "mlmq_serve_request": (_lib.ServeRequest, ["src", "dist_out"]),
+        "mlmq_serve_result": (_lib.ServeResult, ["status", "retries"]),
+        "mlmq_serve_stats": (_lib.ServeStats, ["requests", "peak_inflight", "reserved"]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mlmq_serve.h"', "int main(void){"]
+    for cname, (cls, fields) in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in fields or []:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for cname, (cls, fields) in structs.items():
+        assert int(out[cname]) == ctypes.sizeof(cls), cname
+        for f in fields or []:
+            assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
