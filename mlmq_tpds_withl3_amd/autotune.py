@@ -1,0 +1,106 @@
+"""Parameter search for batched and served queries.
+
+The solver's speed on a graph depends on the bucket width (delta), on how
+many waves work on one query, and on the L1 pop size; the best values differ
+between road-like graphs (long hop diameter, weights up to 1e5) and power-law
+graphs (hubs, weights up to 1e3).  These routines measure candidates on the
+GPU with the graph's own queries and keep the fastest, one coordinate at a
+time (a single pass of coordinate descent from the defaults).  Every
+measurement is an ordinary exact solve: a candidate whose queries fail is
+discarded.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+
+def _delta_grid(base: int) -> list[int]:
+    return sorted({max(1, int(base * f)) for f in (0.25, 0.5, 1.0, 2.0, 4.0)})
+
+
+def default_delta(row, w) -> int:
+    """Starting bucket width: 8 mean edge weights, at least 1."""
+    w = np.asarray(w)
+    return max(1, int(8 * float(w.mean()))) if len(w) else 1
+
+
+def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int = 64,
+                 repeats: int = 1, log=None) -> dict:
+    """Tune one resident graph of a ``serve.Server`` for throughput: each
+    candidate serves a burst of `burst` requests (all arriving at once, drawn
+    from `sources`) and is scored by completed queries per second."""
+    rng = np.random.default_rng(7)
+    src = np.asarray(sources, dtype=np.int32)[rng.integers(0, len(sources), size=burst)]
+    arrival = np.zeros(burst)
+    gsel = np.full(burst, gid, dtype=np.int32)
+    cur = dict(base or {})
+    cur.setdefault("delta", 200000)
+
+    def score(params):
+        server.set_graph_params(gid, **params)
+        best = 0.0
+        for _ in range(repeats):
+            res, st = server.run(arrival, gsel, src)
+            if st["completed"] != burst or st["mismatched"]:
+                return 0.0
+            best = max(best, st["completed_qps"])
+        return best
+
+    best_s = score(cur)
+    trail = [(dict(cur), best_s)]
+    space = [("delta", _delta_grid(cur["delta"])), ("workgroups", [1, 2, 4, 8]),
+             ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64])]
+    for key, values in space:
+        for v in values:
+            if cur.get(key) == v:
+                continue
+            cand = dict(cur, **{key: v})
+            s = score(cand)
+            trail.append((cand, s))
+            if log:
+                log(f"tune graph {gid}: {cand} -> {s:.1f} q/s")
+            if s > best_s:
+                best_s, cur = s, cand
+    server.set_graph_params(gid, **cur)
+    return {"params": cur, "qps": best_s, "trail": trail}
+
+
+def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
+    """Tune a batched launch (``Solver.solve_batch``) over the same sources:
+    scored by batch wall time."""
+    from ._lib import MLMQError
+    from .solver import default_params
+
+    cur = dict(base or {})
+    cur.setdefault("delta", 200000)
+
+    def score(params):
+        p = default_params(timeout_s=60.0, **params)
+        try:
+            solver.solve_batch(sources, p)
+            t0 = time.perf_counter()
+            solver.solve_batch(sources, p)
+        except MLMQError:
+            return float("inf")
+        return time.perf_counter() - t0
+
+    best_t = score(cur)
+    trail = [(dict(cur), best_t)]
+    space = [("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
+             ("l1_pop", [16, 32, 64]), ("buckets", [8, 16, 32])]
+    for key, values in space:
+        for v in values:
+            if cur.get(key) == v:
+                continue
+            cand = dict(cur, **{key: v})
+            if key == "buckets":
+                cand["bucket_window"] = min(cand.get("bucket_window", 4), v)
+            t = score(cand)
+            trail.append((cand, t))
+            if log:
+                log(f"tune batch: {cand} -> {t * 1e3:.1f} ms")
+            if t < best_t:
+                best_t, cur = t, cand
+    return {"params": cur, "batch_s": best_t, "trail": trail}
