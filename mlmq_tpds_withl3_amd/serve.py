@@ -55,14 +55,15 @@ class Server:
 
     # -- graphs -----------------------------------------------------------
     def add_graph(self, graph, **params) -> int:
-        """Make a ``HostGraph`` (or a .gr path) resident; returns its id."""
+        """Make a graph resident: a ``HostGraph``, a .gr path, or a CSR tuple
+        (row int64[n+1], col int32[m], w int32[m]).  Returns its id."""
         p = graph_params(**params)
         gid = ctypes.c_int32()
         if isinstance(graph, str):
             _check(self._L.mlmq_server_add_gr(self._h, graph.encode(), ctypes.byref(p),
                                               ctypes.byref(gid)), self._h)
         else:
-            row, col, w = graph.arrays()
+            row, col, w = graph if isinstance(graph, tuple) else graph.arrays()
             row = np.ascontiguousarray(row, dtype=np.int64)
             col = np.ascontiguousarray(col, dtype=np.int32)
             w = np.ascontiguousarray(w, dtype=np.int32)

@@ -49,7 +49,7 @@ def graphs():
     a = G(*random_csr(3000, 24000, seed=11, wmax=1000))
     b = G(*grid_csr(60, 80, seed=12))
     hg = HostGraph.knn(20000, 3, seed=13, weight_scale=1e5)
-    c = G(*hg.arrays())
+    c = G(*(x.copy() for x in hg.arrays()))   # the views die with hg
     hg.close()
     return [a, b, c]
 
@@ -59,9 +59,7 @@ def new_server(graphs, budget=256 * MiB, **cfg):
     s = Server(0, hbm_budget=budget, **cfg)
     ids = []
     for g in graphs:
-        hg = HostGraph.from_csr(g.row, g.col, g.w)
-        ids.append(s.add_graph(hg, delta=2000 if g.n != 20000 else 200000))
-        hg.close()
+        ids.append(s.add_graph((g.row, g.col, g.w), delta=2000 if g.n != 20000 else 200000))
     return s, ids
 
 
