@@ -57,14 +57,14 @@ extern "C" {
 
 /* Queue modes (values follow the reference's enum mlmq_type,
  * core/include/common.h:85-108). */
-#define MLMQ_QUEUE_L1V_L2DQ     0    /* not implemented */
+#define MLMQ_QUEUE_L1V_L2DQ     0    /* L1 is a FIFO in LDS (l1_vector_queue), same spill rule */
 #define MLMQ_QUEUE_L1N_L2DQ     1    /* no L1: every push and pop goes through L2 */
 #define MLMQ_QUEUE_L1SLF_L2DQ  19    /* the reference's default path */
 
 typedef struct mlmq_ctx mlmq_ctx;
 
 typedef struct mlmq_params {
-    int32_t  queue_mode;      /* MLMQ_QUEUE_L1SLF_L2DQ (default) or MLMQ_QUEUE_L1N_L2DQ */
+    int32_t  queue_mode;      /* MLMQ_QUEUE_L1SLF_L2DQ (default), _L1V_L2DQ or _L1N_L2DQ */
     uint32_t delta;           /* bucket width; reference BENCH_DELTA, default 200000 */
     int32_t  work_blocks;     /* persistent workgroups; 0 = automatic (blocks_per_cu x CUs if
                                  blocks_per_cu > 0, else n / 100000 clamped to [32, 128]) */

@@ -20,7 +20,7 @@ ERRORS = {
     -5: "MLMQ_E_IO", -6: "MLMQ_E_QUEUE_OVERFLOW", -7: "MLMQ_E_TIMEOUT",
     -8: "MLMQ_E_INTERNAL", -9: "MLMQ_E_UNSUPPORTED", -10: "MLMQ_E_REJECTED",
 }
-QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1}
+QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1, "L1V_L2DQ": 0}
 
 # Every exported symbol the headers declare (checked by the CPU test suite).
 SOLVER_SYMBOLS = [

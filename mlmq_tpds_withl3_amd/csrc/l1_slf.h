@@ -79,11 +79,13 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
 
 // Push K registers of records: record k of lane i is present when bit i of
 // mask[k] is set; order is (k, lane).  Requires size <= spill on entry and
-// spill + K * 64 <= kL1Cap.
+// spill + K * 64 <= kL1Cap.  With `fifo` set every record goes to the back:
+// the reference's l1_vector_queue (core/src/l1_queue.cu:74-99), a FIFO with
+// the same spill-all rule.
 template <int K>
 __device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
-                        const uint64_t (&mask)[K], uint32_t spill, const L2Queue &q,
-                        WaveStats &st) {
+                        const uint64_t (&mask)[K], uint32_t spill, uint32_t fifo,
+                        const L2Queue &q, WaveStats &st) {
     l1_uniform(l1);
     uint32_t total = 0;
     // Pivot: the head's distance, or on an empty deque the first pushed
@@ -95,8 +97,9 @@ __device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
         total += popc(mask[k]);
     }
     if (total == 0) return true;
-    uint32_t pivot = 0;
-    if (l1.size) {
+    uint32_t pivot = 0;   // stays 0 in FIFO mode: no record is below it
+    if (fifo) {
+    } else if (l1.size) {
         pivot = uniform(rec_dist(ring[l1.head]));
     } else {
 #pragma unroll

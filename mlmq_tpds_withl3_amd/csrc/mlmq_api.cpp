@@ -455,7 +455,8 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     mlmq_params p;
     if (params) p = *params;
     else mlmq_params_default(&p);
-    const bool slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ;
+    const bool fifo = p.queue_mode == MLMQ_QUEUE_L1V_L2DQ;
+    const bool slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ || fifo;
     if (!slf && p.queue_mode != MLMQ_QUEUE_L1N_L2DQ)
         return fail(c, MLMQ_E_UNSUPPORTED, "queue mode not implemented");
     if (p.delta == 0 || p.delta > 0x7FFFFFFFu) return fail(c, MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
@@ -530,6 +531,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
         L.grid_blocks = blocks;
         L.slf = slf;
+        L.fifo = fifo;
         L.profile = p.profile != 0;
 
         const auto s0 = clk::now();
@@ -707,6 +709,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
         L.grid_blocks = blocks;
         L.slf = true;
+        L.fifo = false;
         L.profile = false;
         mlmq::BatchLaunch B;
         B.ctl_all = c->b_ctl;

@@ -4,7 +4,7 @@
 //   mlmq -i graph.gr [-n 1] [-d delta]
 //   env BENCH_SOURCE (default 0), BENCH_DELTA (default 200000),
 //       BENCH_WARMUPS (default 2), BENCH_REPEATS (default 3),
-//       MLMQ_WORK_BLOCKS (0 = automatic), BENCH_QUEUE (L1SLF_L2DQ | L1N_L2DQ),
+//       MLMQ_WORK_BLOCKS (0 = automatic), BENCH_QUEUE (L1SLF_L2DQ | L1V_L2DQ | L1N_L2DQ),
 //       L3_SUPPLEMENT_ORACLE (int32 .i32 file of expected distances).
 // Prints one "BENCH ..." line per run.  With an oracle file, every run is
 // checked per vertex (WIDE_ORACLE line, SSSP/supplement_oracle.h:6-14) and a
@@ -80,6 +80,7 @@ int main(int argc, char **argv) {
     std::string qname = queue ? queue : "L1SLF_L2DQ";
     if (qname == "L1SLF_L2DQ") p.queue_mode = MLMQ_QUEUE_L1SLF_L2DQ;
     else if (qname == "L1N_L2DQ") p.queue_mode = MLMQ_QUEUE_L1N_L2DQ;
+    else if (qname == "L1V_L2DQ") p.queue_mode = MLMQ_QUEUE_L1V_L2DQ;
     else {
         std::fprintf(stderr, "unsupported BENCH_QUEUE=%s\n", qname.c_str());
         return 2;

@@ -493,6 +493,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.timeout_ticks = timeout_ticks;
         L.grid_blocks = (int)(nb * (uint32_t)g.p.workgroups);
         L.slf = true;
+        L.fifo = false;
         L.profile = false;
         mlmq::ServeLaunch S;
         S.reqs = l.d_reqs;

@@ -48,7 +48,8 @@ struct SolverLaunch {
     uint32_t donate_min;
     uint64_t timeout_ticks;
     int grid_blocks;
-    bool slf;
+    bool slf;       // L1 in LDS (SLF deque or FIFO); false: L2 only
+    bool fifo;      // with slf: L1 is a FIFO (queue mode L1V_L2DQ)
     bool profile;   // per-phase clock build of the kernel
 };
 

@@ -35,6 +35,7 @@ struct SolveArgs {
     uint32_t pop_max;       // records taken from L1 per batch (<= 64)
     uint32_t spill;         // L1 spill threshold (<= kL1Cap - 7 * 64)
     uint32_t donate_min;    // donate to L2 while waves are idle and L1 holds more (0 = off)
+    uint32_t fifo;          // 1: L1 is a FIFO (queue mode L1V_L2DQ), 0: SLF deque
     uint64_t timeout_ticks;
 };
 
@@ -71,7 +72,7 @@ __device__ __forceinline__ bool push_one(uint64_t *ring, L1State &l1, uint64_t r
     if (kSLF) {
         const uint64_t r[1] = {rec};
         const uint64_t m[1] = {im};
-        return l1_push<1>(ring, l1, r, m, a.spill, a.q, st);
+        return l1_push<1>(ring, l1, r, m, a.spill, a.fifo, a.q, st);
     }
     return l2_push(a.q, rec, improved, st);
 }
@@ -149,7 +150,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         st.v[kStatImprove] += improved;
         if (improved) {
             if (kSLF) {
-                ok = l1_push<kHeavyUnroll>(ring, l1, out, im, a.spill, a.q, st);
+                ok = l1_push<kHeavyUnroll>(ring, l1, out, im, a.spill, a.fifo, a.q, st);
             } else {
 #pragma unroll
                 for (int j = 0; j < kHeavyUnroll; ++j)
@@ -235,7 +236,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     bool ok = true;
     if (improved) {
         if (kSLF) {
-            ok = l1_push<kInlineEdges>(ring, l1, out, im, a.spill, a.q, st);
+            ok = l1_push<kInlineEdges>(ring, l1, out, im, a.spill, a.fifo, a.q, st);
         } else {
 #pragma unroll
             for (int k = 0; k < (int)kInlineEdges; ++k)
@@ -709,6 +710,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.pop_max = L.pop_max;
     a.spill = L.spill;
     a.donate_min = L.donate_min;
+    a.fifo = L.fifo ? 1u : 0u;
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }

@@ -56,7 +56,7 @@ def test_fixture_golden_and_source_switch(solver):
         assert np.array_equal(solver.dist(), exp), f"source {src}"
 
 
-@pytest.mark.parametrize("queue", ["L1SLF_L2DQ", "L1N_L2DQ"])
+@pytest.mark.parametrize("queue", ["L1SLF_L2DQ", "L1N_L2DQ", "L1V_L2DQ"])
 @pytest.mark.parametrize("seed", range(6))
 def test_random_graphs(solver, seed, queue):
     say(f"random seed={seed} queue={queue}")
@@ -76,6 +76,8 @@ def test_random_graphs(solver, seed, queue):
     dict(work_blocks=3, delta=1000),
     dict(blocks_per_cu=4, delta=5000),
     dict(queue="L1N_L2DQ", delta=2000),
+    dict(queue="L1V_L2DQ", delta=2000),
+    dict(queue="L1V_L2DQ", l1_spill=8, l1_pop=16),               # FIFO spilling often
 ])
 def test_grid_param_corners(solver, cfg):
     say(f"grid 120x150 {cfg}")
@@ -94,7 +96,7 @@ def test_road264k(solver):
     say("road264k generate")
     g, src, spec = make_workload("road264k")
     row, col, w = g.arrays()
-    for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ")):
+    for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ")):
         say(f"road264k {cfg}")
         t, work, reached = solve_check(solver, row, col, w, src, **cfg)
         assert t["relaxations"] >= work
@@ -166,7 +168,7 @@ def test_errors_and_recovery(solver):
         solver.solve(0, buckets=0)
     assert e.value.code == -1
     with pytest.raises(MLMQError) as e:
-        solver.solve(0, queue_mode=0)
+        solver.solve(0, queue_mode=3)   # L1V_L2V: not on this path
     assert e.value.code == -9
     # watchdog forced from the host: every wave sees the deadline passed and exits
     with pytest.raises(MLMQError) as e:
