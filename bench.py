@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--p99-bound-ms", type=float, default=1000.0,
                     help="serve mode: value = best completed rate with p99 under this bound")
     ap.add_argument("--tune", type=int, default=1, help="batch/serve modes: run the autotuner")
+    ap.add_argument("--extended-sweep", type=int, default=1,
+                    help="single mode: also report the best delta outside the configured sweep")
     ap.add_argument("--batch-waves", type=int, default=8)
     ap.add_argument("--source-seed", type=int, default=12345)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -114,7 +116,7 @@ def main():
     row, col, w = g.arrays()
     log(f"workload {args.workload}: n={g.n} m={g.m} src={src} gen {time.time() - t0:.1f}s")
     if args.l1_pop is None:
-        args.l1_pop = 64
+        args.l1_pop = 32
 
     def make_params(delta):
         return default_params(queue=args.queue, delta=delta, work_blocks=args.work_blocks,
@@ -127,15 +129,26 @@ def main():
     solver.load_csr(row, col, w)
     shape = solver.device_shape()
     sweep = {}
+    extended = None
     if args.delta is not None:
         delta = args.delta
+        params = make_params(delta)
     else:
-        # the workload's delta sweep, untimed: two solves per delta, keep the best
-        for dl in spec["delta_sweep"]:
-            sweep[dl] = min(solver.solve(src, make_params(dl))["device_ms"] for _ in range(2))
-        delta = min(sweep, key=sweep.get)
-        log(f"delta sweep (device ms): {sweep} -> {delta}")
-    params = make_params(delta)
+        # The workload's delta sweep crossed with a small grid of L1 pop size,
+        # L2 top-up threshold and grid size (untimed, best of two solves each).
+        from mlmq_tpds_withl3_amd.autotune import tune_single
+        tuned = tune_single(solver, src, spec["delta_sweep"])
+        for prm, ms in tuned["table"]:
+            sweep[prm["delta"]] = min(ms, sweep.get(prm["delta"], ms))
+        best = tuned["params"]
+        delta = best["delta"]
+        args.l1_pop, args.min_read, args.work_blocks = best["l1_pop"], best["min_read"], best["work_blocks"]
+        params = make_params(delta)
+        log(f"tuned over delta {list(spec['delta_sweep'])}: {best} {tuned['device_ms']:.3f} ms")
+        if args.extended_sweep:
+            ext = tune_single(solver, src, (2_000_000, 20_000_000))
+            extended = {"params": ext["params"], "device_ms": round(ext["device_ms"], 4)}
+            log(f"outside the configured sweep: {extended}")
     for i in range(args.warmup):
         t = solver.solve(src, params)
         log(f"warmup {i}: solve_ms={t['solve_ms']:.3f} device_ms={t['device_ms']:.3f}")
@@ -201,6 +214,7 @@ def main():
             "seed": spec.get("seed"),
             "queue": args.queue, "delta": delta,
             "delta_sweep_device_ms": {str(k): round(v, 4) for k, v in sweep.items()},
+            "outside_sweep_best": extended,
             "buckets": args.buckets, "bucket_window": args.window,
             "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop, "l1_donate": args.l1_donate,
             "work_blocks": last["work_blocks"], "waves": last["waves"],
@@ -261,10 +275,17 @@ def run_batch(args):
     row, col, w = g.arrays()
     sources = np.random.default_rng(args.source_seed).integers(0, g.n, size=args.queries)
     delta = args.delta if args.delta is not None else spec["delta"]
-    params = default_params(delta=delta, batch_waves=args.batch_waves, work_blocks=args.work_blocks,
-                            l1_pop=args.l1_pop, timeout_s=60.0)
+    base = dict(delta=delta, batch_waves=args.batch_waves, work_blocks=args.work_blocks,
+                l1_pop=args.l1_pop)
     solver = Solver(0)
     solver.load_csr(row, col, w)
+    tuned = None
+    if args.tune and args.delta is None:
+        from mlmq_tpds_withl3_amd.autotune import tune_batch
+        tuned = tune_batch(solver, sources, base=base, log=log)
+        base = tuned["params"]
+        log(f"tuned batch parameters: {base} ({tuned['batch_s'] * 1e3:.1f} ms per batch)")
+    params = default_params(timeout_s=60.0, **base)
     for _ in range(args.warmup):
         solver.solve_batch(sources, params)
     times = []
@@ -278,8 +299,14 @@ def run_batch(args):
         refs = list(ex.map(lambda s: O.dijkstra(row, col, w, int(s)), sources))
     correct = all(np.array_equal(solver.batch_dist(i), refs[i][0]) for i in range(len(sources)))
     e_r = sum(r[1] for r in refs)
-    # comparator: the same queries one at a time (plain solves)
+    # comparator: the same queries one at a time (plain solves, parameters
+    # tuned for single queries the same way as the single-query mode)
     sp = default_params(delta=delta, timeout_s=60.0)
+    if args.tune:
+        from mlmq_tpds_withl3_amd.autotune import tune_single
+        st_ = tune_single(solver, int(sources[0]), spec["delta_sweep"])
+        sp = default_params(timeout_s=60.0, **st_["params"])
+        log(f"one-at-a-time parameters: {st_['params']}")
     solver.solve(int(sources[0]), sp)
     s0 = time.perf_counter()
     seq_ms = [solver.solve(int(s), sp)["solve_ms"] for s in sources]
@@ -294,7 +321,8 @@ def run_batch(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "uint32", "data": "synthetic",
         "config": {"workload": workload, "mode": "batch", "queries": args.queries,
                    "vertices": g.n, "edges": g.m, "source_seed": args.source_seed,
-                   "delta": delta, "batch_waves": last["waves_per_query"],
+                   "delta": params.delta, "tuned": base if tuned else None,
+                   "batch_waves": last["waves_per_query"],
                    "work_blocks": last["work_blocks"], "graph_replay": last["graph_replay"],
                    "bucket_capacity": last["bucket_capacity"]},
         "gteps": round(e_r * args.steps / wall / 1e9, 4),
@@ -356,7 +384,9 @@ def run_serve(args):
         # one-at-a-time comparator: plain solves of the pool sources
         with Solver(0) as sv:
             sv.load_csr(row, col, w)
-            sp = default_params(delta=int(tuned[-1]["delta"]), timeout_s=60.0)
+            from mlmq_tpds_withl3_amd.autotune import tune_single
+            sp = default_params(timeout_s=60.0,
+                                **tune_single(sv, pool[0], spec["delta_sweep"])["params"])
             sv.solve(pool[0], sp)
             t1 = time.perf_counter()
             for s_ in pool:

@@ -77,7 +77,7 @@ typedef struct mlmq_params {
     double   timeout_s;       /* device watchdog; <= 0 means 60 s */
     int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
     int32_t  profile;         /* 1: per-phase clock build of the kernel (mlmq_timing.phase_cycles) */
-    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 64 */
+    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 32 */
     int32_t  l1_spill;        /* L1 holding more records than this after a push spills to L2;
                                  1..64, default 64 */
     int32_t  l1_donate;       /* while some waves are idle, a wave whose L1 holds more than this
@@ -86,7 +86,9 @@ typedef struct mlmq_params {
                                  default 8 */
     int32_t  use_graph;       /* mlmq_solve_batch: replay the launch as a captured hipGraph
                                  (default 1) */
-    int32_t  reserved[1];
+    int32_t  hub_split;       /* 1: a popped hub (> 2048 out-edges) keeps its first 1024-edge
+                                 chunk and sends the rest to L2 as chunk records that other
+                                 waves relax; 0 (default): one wave relaxes the whole list */
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -113,9 +115,10 @@ typedef struct mlmq_timing {
     uint64_t phase_cycles[8];
     /* params.profile = 1 only: cycles in owner computation, edge read (incl.
      * the stale check wait), atomicMin round trip; 16 empty laps (clock
-     * overhead); edge chunks processed */
+     * overhead); [4] unused */
     uint64_t detail_cycles[5];
     uint64_t donated;         /* records moved from L1 to L2 for idle waves */
+    uint64_t hub_chunks;      /* chunk records pushed by hub splitting */
 } mlmq_timing;
 
 /* Statistics of one mlmq_solve_batch call. */

@@ -74,7 +74,8 @@ typedef struct mlmq_serve_graph_params {
     int64_t  bucket_capacity; /* ring slots per bucket (rounded up to 2^k); 0 = n / 32, >= 8192.
                                  Doubled automatically when a request overflows (the request
                                  is then run again). */
-    int32_t  reserved[8];
+    int32_t  hub_split;       /* 1: split hub edge lists into chunk records (mlmq_params.hub_split) */
+    int32_t  reserved[7];
 } mlmq_serve_graph_params;
 
 typedef struct mlmq_serve_request {

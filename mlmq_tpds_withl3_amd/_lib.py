@@ -59,7 +59,7 @@ class Params(ctypes.Structure):
         ("auto_retry", ctypes.c_int32), ("profile", ctypes.c_int32),
         ("l1_pop", ctypes.c_int32), ("l1_spill", ctypes.c_int32),
         ("l1_donate", ctypes.c_int32), ("batch_waves", ctypes.c_int32),
-        ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
+        ("use_graph", ctypes.c_int32), ("hub_split", ctypes.c_int32),
     ]
 
 
@@ -75,8 +75,9 @@ class Timing(ctypes.Structure):
         ("batches", ctypes.c_uint64), ("idle_polls", ctypes.c_uint64),
         ("retries", ctypes.c_uint64), ("phase_cycles", ctypes.c_uint64 * 8),
         ("detail_cycles", ctypes.c_uint64 * 5), ("donated", ctypes.c_uint64),
+        ("hub_chunks", ctypes.c_uint64),
     ]
-    DETAILS = ("owner", "edge_read", "atomic", "laps16", "chunks")
+    DETAILS = ("owner", "edge_read", "atomic", "laps16", "unused")
     PHASES = ("l1_pop", "l2_pop", "row_fetch", "edges", "flush", "idle", "total", "credit")
 
     def as_dict(self) -> dict:
@@ -131,7 +132,7 @@ class ServeGraphParams(ctypes.Structure):
         ("min_read", ctypes.c_int32), ("l1_pop", ctypes.c_int32),
         ("l1_spill", ctypes.c_int32), ("workgroups", ctypes.c_int32),
         ("waves", ctypes.c_int32), ("bucket_capacity", ctypes.c_int64),
-        ("reserved", ctypes.c_int32 * 8),
+        ("hub_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
     ]
 
 

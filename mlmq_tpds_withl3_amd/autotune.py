@@ -26,6 +26,29 @@ def default_delta(row, w) -> int:
     return max(1, int(8 * float(w.mean()))) if len(w) else 1
 
 
+def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None) -> dict:
+    """Grid search for single queries: every delta in `deltas` crossed with
+    `grid` (a list of parameter dicts, default: L1 pop size / L2 top-up
+    threshold pairs and two grid sizes).  Each point is the best device time
+    of `repeats` solves.  Returns the best point and the whole table."""
+    from .solver import default_params
+
+    if grid is None:
+        grid = [dict(l1_pop=pop, min_read=mr, work_blocks=wb)
+                for wb in (64, 128) for pop, mr in ((32, 32), (32, 8), (64, 8))]
+    table = []
+    for d in deltas:
+        for g in grid:
+            params = dict(g, delta=int(d))
+            p = default_params(timeout_s=30.0, **params)
+            ms = min(solver.solve(src, p)["device_ms"] for _ in range(repeats))
+            table.append((params, ms))
+            if log:
+                log(f"tune single: {params} -> {ms:.3f} ms")
+    best = min(table, key=lambda t: t[1])
+    return {"params": best[0], "device_ms": best[1], "table": table}
+
+
 def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int = 64,
                  repeats: int = 1, log=None) -> dict:
     """Tune one resident graph of a ``serve.Server`` for throughput: each

@@ -8,8 +8,17 @@
 // maximum degree is 9) is fully described by its block, so relaxing it needs
 // no row-offset or column load: one aligned 64-byte load replaces the
 // row -> col dependency chain of CSR.  Heavier vertices keep using the CSR
-// arrays through the start offset.  The blocks are built on the device from
-// the CSR when a graph is loaded; the CSR stays the graph of record.
+// arrays through the start offset; their inline words are zero, except for
+// hubs (more than kSplitDeg out-edges), whose word 2 holds hub id + 1 and
+// word 3 the chunk size their edge list is split into.  The blocks are built
+// on the device from the CSR when a graph is loaded; the CSR stays the graph
+// of record.
+//
+// Hub splitting: a wave that pops a hub keeps the first chunk of its edge
+// list and pushes one chunk record per further chunk to L2, where idle waves
+// take them.  A chunk record has the hub's distance and, in the vertex word,
+// kChunkFlag | hub id << kChunkBits | chunk index; the hub table maps hub ids
+// back to vertices.  Vertex ids are below 2^31, so the flag bit is free.
 #pragma once
 
 #include "device_common.h"
@@ -17,6 +26,15 @@
 namespace mlmq {
 
 constexpr uint32_t kInlineEdges = 7;
+constexpr uint32_t kSplitDeg = 2048;        // hubs: more out-edges than this
+constexpr uint32_t kChunkEdges = 1024;      // default chunk of a hub's edge list
+constexpr uint32_t kChunkFlag = 0x80000000u;
+constexpr uint32_t kChunkBits = 16;         // chunk index bits (chunks per hub < 2^16)
+constexpr uint32_t kMaxHubs = 1u << 15;     // hub id bits: 15
+
+__device__ __host__ inline uint32_t chunk_vertex_word(uint32_t hub, uint32_t ci) {
+    return kChunkFlag | (hub << kChunkBits) | ci;
+}
 
 struct HopBlock {
     uint4 q[4];

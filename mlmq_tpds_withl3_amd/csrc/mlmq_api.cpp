@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "hop_block.h"
 #include "solver_internal.h"
 
 struct mlmq_ctx {
@@ -30,6 +31,8 @@ struct mlmq_ctx {
     int64_t m = 0;
     uint32_t *row = nullptr, *col = nullptr, *wgt = nullptr;
     uint4 *blocks = nullptr;   // hop blocks, always owned by the context
+    uint2 *hubs = nullptr;     // hub table (vertex, chunk size), owned
+    uint32_t nhubs = 0;
     bool owns_graph = false;
     mlmq_graph_info info{};
     // per-query state
@@ -115,6 +118,9 @@ void free_graph(mlmq_ctx *c) {
     }
     if (c->blocks) (void)hipFree(c->blocks);
     c->blocks = nullptr;
+    if (c->hubs) (void)hipFree(c->hubs);
+    c->hubs = nullptr;
+    c->nhubs = 0;
     c->row = c->col = c->wgt = nullptr;
     c->owns_graph = false;
     c->n = 0;
@@ -179,10 +185,41 @@ int validate_csr(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const in
 }
 
 // Device-side hop blocks (64 B per vertex) from the device CSR.
+// Device-side hop blocks (64 B per vertex) from the device CSR, and the hub
+// table: vertices with more than kSplitDeg out-edges, highest degree first,
+// at most kMaxHubs of them, each with a chunk size that keeps its chunk count
+// below 2^kChunkBits.
 int build_blocks(mlmq_ctx *c) {
     HIP_TRY(c, hipMalloc(&c->blocks, 64 * (size_t)c->n));
     HIP_TRY(c, mlmq::launch_build_blocks(c->row, c->col, c->wgt, (uint32_t)c->n, c->blocks, c->stream));
+    std::vector<uint32_t> row((size_t)c->n + 1);
+    HIP_TRY(c, hipMemcpyAsync(row.data(), c->row, sizeof(uint32_t) * row.size(), hipMemcpyDeviceToHost,
+                              c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<std::pair<uint32_t, uint32_t>> hubs;   // (degree, vertex)
+    for (int32_t v = 0; v < c->n; ++v) {
+        const uint32_t deg = row[v + 1] - row[v];
+        if (deg > mlmq::kSplitDeg) hubs.push_back({deg, (uint32_t)v});
+    }
+    std::sort(hubs.begin(), hubs.end(), [](const auto &a, const auto &b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    if (hubs.size() > mlmq::kMaxHubs) hubs.resize(mlmq::kMaxHubs);
+    if (!hubs.empty()) {
+        std::vector<uint2> tab(hubs.size());
+        const uint32_t max_chunks = (1u << mlmq::kChunkBits) - 1;
+        for (size_t h = 0; h < hubs.size(); ++h) {
+            uint32_t cs = mlmq::kChunkEdges;
+            const uint32_t deg = hubs[h].first;
+            if ((deg + cs - 1) / cs > max_chunks) cs = ((deg + max_chunks - 1) / max_chunks + 255) & ~255u;
+            tab[h] = make_uint2(hubs[h].second, cs);
+        }
+        HIP_TRY(c, hipMalloc(&c->hubs, sizeof(uint2) * tab.size()));
+        HIP_TRY(c, hipMemcpy(c->hubs, tab.data(), sizeof(uint2) * tab.size(), hipMemcpyHostToDevice));
+        c->nhubs = (uint32_t)tab.size();
+        HIP_TRY(c, mlmq::launch_patch_hubs(c->hubs, c->nhubs, c->blocks, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return MLMQ_OK;
 }
 
@@ -259,6 +296,7 @@ int mlmq::ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v) {
     v->col = c->col;
     v->wgt = c->wgt;
     v->blocks = c->blocks;
+    v->hubs = c->hubs;
     v->n = c->n;
     v->m = c->m;
     v->device = c->device;
@@ -286,11 +324,12 @@ void mlmq_params_default(mlmq_params *p) {
     p->timeout_s = 60.0;
     p->auto_retry = 1;
     p->profile = 0;
-    p->l1_pop = 64;
+    p->l1_pop = 32;
     p->l1_spill = 64;
     p->l1_donate = 0;
     p->batch_waves = 8;
     p->use_graph = 1;
+    p->hub_split = 0;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -513,6 +552,8 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.col = c->col;
         L.wgt = c->wgt;
         L.blocks = c->blocks;
+        L.hubs = c->hubs;
+        L.split = p.hub_split != 0;
         L.dist = c->dist;
         L.slots = c->slots;
         L.ctl = c->ctl;
@@ -522,7 +563,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.cap_log2 = cap_log2;
         L.delta = p.delta;
         L.batch = (uint32_t)p.l2_batch;
-        L.spin_limit = 1u << 22;
+        L.spin_limit = mlmq::kSpinLimit;
         L.min_read = (uint32_t)p.min_read;
         L.max_sleep = 16;
         L.pop_max = (uint32_t)p.l1_pop;
@@ -593,8 +634,10 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
             timing->idle_polls = st[mlmq::kStatIdle];
             timing->retries = retries;
             for (int i = 0; i < 8; ++i) timing->phase_cycles[i] = st[mlmq::kClkL1Pop + i];
-            for (int i = 0; i < 5; ++i) timing->detail_cycles[i] = st[mlmq::kClkOwner + i];
+            for (int i = 0; i < 4; ++i) timing->detail_cycles[i] = st[mlmq::kClkOwner + i];
+            timing->detail_cycles[4] = 0;
             timing->donated = st[mlmq::kStatDonated];
+            timing->hub_chunks = st[mlmq::kStatChunks];
         }
         return MLMQ_OK;
     }
@@ -691,6 +734,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.col = c->col;
         L.wgt = c->wgt;
         L.blocks = c->blocks;
+        L.hubs = c->hubs;
+        L.split = p.hub_split != 0;
         L.dist = nullptr;
         L.slots = nullptr;
         L.ctl = nullptr;
@@ -700,7 +745,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.cap_log2 = cap_log2;
         L.delta = p.delta;
         L.batch = (uint32_t)p.l2_batch;
-        L.spin_limit = 1u << 22;
+        L.spin_limit = mlmq::kSpinLimit;
         L.min_read = (uint32_t)p.min_read;
         L.max_sleep = 16;
         L.pop_max = (uint32_t)p.l1_pop;
@@ -730,7 +775,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             (uint64_t)p.min_read, (uint64_t)p.l1_pop, (uint64_t)p.l1_spill, (uint64_t)p.l1_donate,
             L.timeout_ticks, (uint64_t)(uintptr_t)c->b_dist, (uint64_t)(uintptr_t)c->b_slots,
             (uint64_t)(uintptr_t)c->b_ctl, (uint64_t)(uintptr_t)c->b_sources,
-            (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks};
+            (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks,
+            (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

@@ -121,6 +121,7 @@ int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n,
     if (p.workgroups < 1 || p.workgroups > 64) return fail(s, MLMQ_E_INVALID, "workgroups must be in [1, 64]");
     if (p.waves != 4 && p.waves != 8 && p.waves != 16) return fail(s, MLMQ_E_INVALID, "waves must be 4, 8 or 16");
     if (p.bucket_capacity < 0) return fail(s, MLMQ_E_INVALID, "negative bucket_capacity");
+    if (p.hub_split != 0 && p.hub_split != 1) return fail(s, MLMQ_E_INVALID, "hub_split must be 0 or 1");
     uint64_t cap = (uint64_t)p.bucket_capacity;
     if (cap == 0) cap = std::max<uint64_t>((uint64_t)n / 32, 8192);
     const uint32_t c = ceil_log2(cap);
@@ -478,13 +479,15 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.col = g.v.col;
         L.wgt = g.v.wgt;
         L.blocks = g.v.blocks;
+        L.hubs = g.v.hubs;
+        L.split = g.p.hub_split != 0;
         L.n = (uint32_t)g.v.n;
         L.nbuckets = (uint32_t)g.p.buckets;
         L.window = (uint32_t)g.p.bucket_window;
         L.cap_log2 = g.cap_log2;
         L.delta = g.p.delta;
         L.batch = (uint32_t)g.p.l2_batch;
-        L.spin_limit = 1u << 22;
+        L.spin_limit = mlmq::kSpinLimit;
         L.min_read = (uint32_t)g.p.min_read;
         L.max_sleep = 16;
         L.pop_max = (uint32_t)g.p.l1_pop;

@@ -16,6 +16,7 @@ struct DeviceGraphView {
     const uint32_t *col;
     const uint32_t *wgt;
     const uint4 *blocks;
+    const uint2 *hubs;
     int32_t n;
     int64_t m;
     int device;
@@ -26,11 +27,18 @@ int ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v);
 
 constexpr int kWavesPerBlock = 4;   // 256-thread persistent workgroups
 
+// Bounded wait of an L2 writer on a slot whose previous record is not yet
+// taken: ~8k CAS round trips (about 10 ms) before the push reports overflow.
+// A reader that has claimed a slot empties it within microseconds, so a
+// longer wait only delays the overflow rerun of a genuinely full ring.
+constexpr uint32_t kSpinLimit = 1u << 13;
+
 struct SolverLaunch {
     const uint32_t *row;
     const uint32_t *col;
     const uint32_t *wgt;
     const uint4 *blocks;
+    const uint2 *hubs;     // hub table (vertex, chunk size); null: no hubs
     uint32_t *dist;
     uint64_t *slots;
     uint32_t *ctl;
@@ -50,6 +58,7 @@ struct SolverLaunch {
     int grid_blocks;
     bool slf;       // L1 in LDS (SLF deque or FIFO); false: L2 only
     bool fifo;      // with slf: L1 is a FIFO (queue mode L1V_L2DQ)
+    bool split;     // split hub edge lists into chunk records
     bool profile;   // per-phase clock build of the kernel
 };
 
@@ -99,6 +108,7 @@ hipError_t launch_reset(uint32_t *dist, uint32_t n, uint32_t src, uint32_t *ctl,
                         hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t count, uint64_t value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);
+hipError_t launch_patch_hubs(const uint2 *hubs, uint32_t nh, uint4 *blocks, hipStream_t s);
 hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
                                uint32_t n, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);

@@ -27,6 +27,7 @@ struct SolveArgs {
     const uint32_t *col;    // m destinations
     const uint32_t *wgt;    // m weights
     const uint4 *blocks;    // n hop blocks (4 x uint4 each)
+    const uint2 *hubs;      // hub table: (vertex, chunk size) per hub id
     uint32_t *dist;         // n distances
     L2Queue q;
     uint32_t n;
@@ -36,6 +37,7 @@ struct SolveArgs {
     uint32_t spill;         // L1 spill threshold (<= kL1Cap - 7 * 64)
     uint32_t donate_min;    // donate to L2 while waves are idle and L1 holds more (0 = off)
     uint32_t fifo;          // 1: L1 is a FIFO (queue mode L1V_L2DQ), 0: SLF deque
+    uint32_t split;         // 1: split hub edge lists into chunk records
     uint64_t timeout_ticks;
 };
 
@@ -161,6 +163,35 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
     return ok;
 }
 
+// Push the chunk records of the hubs this batch split: lane i owns
+// `nextra` chunks (indices 1..nextra) of hub `hub` at distance `d`.  The
+// chunks are spread over the lanes 64 at a time (owner of a slot = running
+// maximum of start markers, as in relax_heavy) and pushed to L2.
+__device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32_t nextra,
+                            uint32_t *map, WaveStats &st) {
+    const uint32_t lane = lane_id();
+    uint32_t total = 0;
+    const uint32_t excl = wave_excl_scan(nextra, total);
+    st.v[kStatChunks] += total;
+    uint32_t carry = 0;
+    for (uint32_t cb = 0; cb < total; cb += kWave) {
+        map[lane] = 0;
+        wave_sync();
+        if (nextra > 0 && excl >= cb && excl - cb < (uint32_t)kWave) map[excl - cb] = lane + 1;
+        wave_sync();
+        const uint32_t o1 = max(carry, wave_incl_max(map[lane]));
+        carry = read_lane(o1, kWave - 1);
+        const bool act = cb + lane < total;
+        const int o = act ? (int)o1 - 1 : 0;
+        const uint32_t oh = (uint32_t)__shfl((int)hub, o, kWave);
+        const uint32_t od = (uint32_t)__shfl((int)d, o, kWave);
+        const uint32_t oe = (uint32_t)__shfl((int)excl, o, kWave);
+        const uint64_t rec = make_rec(chunk_vertex_word(oh, 1 + cb + lane - oe), od);
+        if (!l2_push(a.q, rec, act, st)) return false;
+    }
+    return true;
+}
+
 // Relax all out-edges of the records in lanes [0, nin) (simple_process,
 // SS/SSSP/sssp_run.cu:143-413), one record per lane, in two round trips:
 //   1. the record's current distance (stale filter, reference :174: process
@@ -175,8 +206,18 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
                               uint32_t &hungry, WaveStats &st, PhaseClock<kProf> &clk) {
     const uint32_t lane = lane_id();
     const bool valid = lane < nin;
-    const uint32_t v = rec_vertex(rec);
+    uint32_t v = rec_vertex(rec);
     const uint32_t d = rec_dist(rec);
+    // A chunk record names (hub id, chunk index); one extra round trip to the
+    // hub table, taken only by batches that hold chunks.
+    const bool chunk = valid && (v & kChunkFlag) != 0;
+    uint32_t ci = 0;
+    if (ballot(chunk)) {
+        if (chunk) {
+            ci = v & ((1u << kChunkBits) - 1);
+            v = a.hubs[(v & ~kChunkFlag) >> kChunkBits].x;
+        }
+    }
     uint32_t cur = 0;
     HopBlock b = empty_block();
     if (valid) {
@@ -187,10 +228,25 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     hungry = a.donate_min ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
     const bool keep = valid && d <= cur;
     st.v[kStatStale] += popc(ballot(valid && !keep));
-    const uint32_t deg = keep ? b.deg() : 0u;
-    const bool heavy = deg > kInlineEdges;
+    uint32_t deg = keep ? b.deg() : 0u;
+    uint32_t hstart = b.start();
+    uint32_t nextra = 0;   // chunks of a hub popped here that go to L2
+    const uint32_t hub_word = b.q[0].z, csize = b.q[0].w;
+    if (chunk) {
+        const uint32_t lo = ci * csize;
+        deg = lo < deg ? min(csize, deg - lo) : 0u;
+        hstart += lo;
+    } else if (a.split && deg > kInlineEdges && hub_word != 0) {
+        nextra = (deg - 1) / csize;
+        deg = min(deg, csize);
+    }
+    // chunk lanes always take the CSR path (their edges are not inline)
+    const bool heavy = chunk ? deg > 0 : deg > kInlineEdges;
     const uint32_t ideg = heavy ? 0u : deg;
     clk.lap(st, kClkRow);
+    if (ballot(nextra > 0)) {
+        if (!push_chunks(a, hub_word - 1, d, nextra, map, st)) return false;
+    }
 
     uint32_t c[kInlineEdges], nd[kInlineEdges], pv[kInlineEdges];
 #pragma unroll
@@ -245,7 +301,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
         clk.lap(st, kClkFlush);
     }
     if (ok && ballot(heavy)) {
-        ok = relax_heavy<kSLF>(a, heavy, d, b.start(), deg, ring, l1, map, st);
+        ok = relax_heavy<kSLF>(a, heavy, d, hstart, deg, ring, l1, map, st);
         clk.lap(st, kClkEdges);
     }
     return ok;
@@ -594,6 +650,18 @@ __global__ void build_blocks_kernel(const uint32_t *row, const uint32_t *col, co
     }
 }
 
+// Mark hubs in their hop blocks: word 2 = hub id + 1, word 3 = chunk size.
+__global__ void patch_hubs_kernel(const uint2 *hubs, uint32_t nh, uint4 *blocks) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < nh) {
+        uint4 *p = blocks + 4ull * hubs[h].x;
+        uint4 q = p[0];
+        q.z = h + 1;
+        q.w = hubs[h].y;
+        p[0] = q;
+    }
+}
+
 // Per-query reset: dist = INT_MAX except dist[src] = 0; control block zeroed
 // with tail[0] = 1; the seed record (src, 0) in slot 0 of bucket 0, which is
 // where the reference's write_through of (src, 0) lands at first_pos 0
@@ -662,6 +730,12 @@ hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const u
     return hipGetLastError();
 }
 
+hipError_t launch_patch_hubs(const uint2 *hubs, uint32_t nh, uint4 *blocks, hipStream_t s) {
+    if (nh == 0) return hipSuccess;
+    hipLaunchKernelGGL(patch_hubs_kernel, dim3(grid_for(nh, 256)), dim3(256), 0, s, hubs, nh, blocks);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s) {
     hipLaunchKernelGGL(copy_u32_kernel, dim3(grid_for(count, 256)), dim3(256), 0, s, dst, src,
                        count);
@@ -695,6 +769,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.col = L.col;
     a.wgt = L.wgt;
     a.blocks = L.blocks;
+    a.hubs = L.hubs;
     a.dist = L.dist;
     a.q.slots = L.slots;
     a.q.ctl = L.ctl;
@@ -711,6 +786,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.spill = L.spill;
     a.donate_min = L.donate_min;
     a.fifo = L.fifo ? 1u : 0u;
+    a.split = L.hubs && L.split ? 1u : 0u;
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }

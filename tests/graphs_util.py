@@ -63,3 +63,27 @@ def reachable_work(row, col, src):
                 stack.append(int(v))
     deg = np.diff(row)
     return int(seen.sum()), int(deg[seen].sum())
+
+
+def hub_csr(n, hub_degrees, m_rest, seed, wmax=1000):
+    """Random multigraph whose first len(hub_degrees) vertices are hubs with the
+    given out-degrees (edges to uniform random targets), plus `m_rest` random
+    edges among all vertices; every vertex also links back to hub 0 so that
+    the hubs sit on many shortest paths."""
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for h, deg in enumerate(hub_degrees):
+        src.append(np.full(deg, h))
+        dst.append(rng.integers(0, n, size=deg))
+    src.append(rng.integers(0, n, size=m_rest))
+    dst.append(rng.integers(0, n, size=m_rest))
+    src.append(np.arange(n))
+    dst.append(np.zeros(n, dtype=np.int64))
+    src = np.concatenate(src)
+    dst = np.concatenate(dst)
+    w = rng.integers(1, wmax + 1, size=len(src))
+    order = np.lexsort((dst, src))
+    src, dst, w = src[order], dst[order], w[order]
+    row = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(row, src + 1, 1)
+    return np.cumsum(row), dst.astype(np.int32), w.astype(np.int32)

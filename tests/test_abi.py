@@ -45,7 +45,8 @@ def test_loader_binds_all():
     L.mlmq_params_default(ctypes.byref(p))
     assert (p.queue_mode, p.delta, p.buckets, p.bucket_window, p.l2_batch, p.min_read) == \
         (19, 200000, 16, 4, 8, 8)
-    assert (p.l1_pop, p.l1_spill, p.l1_donate, p.profile) == (64, 64, 0, 0)
+    assert (p.l1_pop, p.l1_spill, p.l1_donate, p.profile) == (32, 64, 0, 0)
+    assert (p.batch_waves, p.use_graph, p.hub_split) == (8, 1, 0)
 
 
 def test_library_is_gfx950_and_independent_of_oracle():
@@ -79,12 +80,12 @@ def test_ctypes_layouts_match_c(tmp_path):
     """Every structure the binding mirrors has the size and field offsets the
     C compiler gives it."""
     structs = {
-        "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph"]),
-        "mlmq_timing": (_lib.Timing, None),
+        "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph", "hub_split"]),
+        "mlmq_timing": (_lib.Timing, ["donated", "hub_chunks"]),
         "mlmq_batch_timing": (_lib.BatchTiming, None),
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
         "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "reserved"]),
-        "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "reserved"]),
+        "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "reserved"]),
         "mlmq_serve_request": (_lib.ServeRequest, ["src", "dist_out"]),
         "mlmq_serve_result": (_lib.ServeResult, ["status", "retries"]),
         "mlmq_serve_stats": (_lib.ServeStats, ["requests", "peak_inflight", "reserved"]),

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from graphs_util import random_csr, grid_csr
+from graphs_util import random_csr, grid_csr, hub_csr
 from mlmq_tpds_withl3_amd import Solver, MLMQError, make_workload
 
 pytestmark = pytest.mark.gpu
@@ -65,6 +65,15 @@ def test_random_batch(solver, waves, blocks):
     rng = np.random.default_rng(waves * 10 + blocks)
     sources = rng.integers(0, 3000, size=40)
     check_batch(solver, row, col, w, sources, batch_waves=waves, work_blocks=blocks, delta=200)
+
+
+@pytest.mark.parametrize("split", [1, 0])
+def test_hub_batch(solver, split):
+    say(f"hub batch split={split}")
+    row, col, w = hub_csr(20000, [40000, 5000, 2100], 40000, seed=8)
+    solver.load_csr(row, col, w)
+    sources = [0, 1, 2, 19999, 0, 7]
+    check_batch(solver, row, col, w, sources, work_blocks=3, hub_split=split, delta=300)
 
 
 def test_graph_replays_interleaved(solver):

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from graphs_util import random_csr, grid_csr
+from graphs_util import hub_csr, random_csr, grid_csr
 from mlmq_tpds_withl3_amd import Solver, HostGraph, MLMQError, make_workload
 from mlmq_tpds_withl3_amd import solver as S
 
@@ -83,6 +83,41 @@ def test_grid_param_corners(solver, cfg):
     say(f"grid 120x150 {cfg}")
     row, col, w = grid_csr(120, 150, 7)
     solve_check(solver, row, col, w, 0, **cfg)
+
+
+HUBS = [70000, 9000, 4097, 2049, 2048, 1500]   # degrees 70002 .. 1504: five hubs split
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(),
+    dict(hub_split=0),
+    dict(queue="L1N_L2DQ"),
+    dict(queue="L1V_L2DQ"),
+    dict(work_blocks=1),                         # one workgroup takes every chunk itself
+    dict(delta=100, l2_batch=64, min_read=64),
+    dict(l1_spill=4, l1_pop=8),
+])
+def test_hub_split(solver, cfg):
+    say(f"hubs {cfg}")
+    row, col, w = hub_csr(30000, HUBS, 60000, seed=5)
+    solver.load_csr(row, col, w)
+    params = dict(hub_split=1)
+    params.update(cfg)
+    for src in (0, 3, 4, 29999):
+        t, _, _ = solve_check(solver, row, col, w, src, load=False, **params)
+        if params["hub_split"]:
+            assert t["hub_chunks"] > 0     # every source reaches the hubs
+        else:
+            assert t["hub_chunks"] == 0
+
+
+def test_hub_chunk_boundaries(solver):
+    """Hub degrees at and around multiples of the 1024-edge chunk."""
+    say("hub chunk boundaries")
+    for degs in ([2049], [2050, 3072, 3073], [1024 * 7 + 1, 5000, 4096]):
+        row, col, w = hub_csr(8000, degs, 16000, seed=len(degs))
+        for src in (0, 7999):
+            solve_check(solver, row, col, w, src, delta=500)
 
 
 def test_zero_weight_components(solver):

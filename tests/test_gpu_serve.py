@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from graphs_util import random_csr, grid_csr
+from graphs_util import random_csr, grid_csr, hub_csr
 from mlmq_tpds_withl3_amd import HostGraph, MLMQError
 from mlmq_tpds_withl3_amd.serve import Server, poisson_trace
 
@@ -164,6 +164,17 @@ def test_workgroups_per_request(graphs, waves, wgs):
     gsel = np.array([0, 1, 2, 2, 1, 0, 2], dtype=np.int32)
     src = np.array([3, 40, 11, 19000, 4000, 2999, 0], dtype=np.int32)
     run_checked(s, graphs, np.linspace(0, 0.005, 7), gsel, src, f"w{waves}x{wgs}")
+    s.close()
+
+
+def test_hub_graph_served(graphs):
+    say("hub graph served")
+    g = G(*hub_csr(20000, [40000, 5000, 2100], 40000, seed=8))
+    s, ids = new_server([g], streams=2, max_batch=4)
+    for split in (1, 0):
+        s.set_graph_params(0, delta=300, workgroups=2, hub_split=split)
+        run_checked(s, [g], np.linspace(0, 0.002, 6), np.zeros(6, dtype=np.int32),
+                    np.array([0, 1, 2, 19999, 5, 0], dtype=np.int32), f"hubs split={split}")
     s.close()
 
 
