@@ -89,6 +89,11 @@ typedef struct mlmq_params {
     int32_t  hub_split;       /* 1: a popped hub (> 2048 out-edges) keeps its first 1024-edge
                                  chunk and sends the rest to L2 as chunk records that other
                                  waves relax; 0 (default): one wave relaxes the whole list */
+    int32_t  stale_check;     /* filter of popped records whose distance has since dropped:
+                                 2 (default) coherent load of dist[v], 1 load through the L2
+                                 (may miss a drop made on another XCD), 0 none. Results are
+                                 exact in every mode; only the work differs */
+    int32_t  reserved[3];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */

@@ -23,6 +23,7 @@ namespace mlmq {
 
 constexpr uint32_t kL1Cap = 512;
 constexpr uint32_t kL1Mask = kL1Cap - 1;
+constexpr uint32_t kL1Alloc = kL1Cap + 64;   // ring + one discard slot per lane
 
 struct L1State {
     uint32_t head;
@@ -77,54 +78,60 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
     return l2_push(q, x, lane_id() < k, st);
 }
 
-// Push K registers of records: record k of lane i is present when bit i of
-// mask[k] is set; order is (k, lane).  Requires size <= spill on entry and
+// Push K registers of records: record k of this lane is present when bit k
+// of `bits` is set.  Requires size <= spill on entry and
 // spill + K * 64 <= kL1Cap.  With `fifo` set every record goes to the back:
 // the reference's l1_vector_queue (core/src/l1_queue.cu:74-99), a FIFO with
 // the same spill-all rule.
+//
+// Positions come from one packed wave prefix scan (front count in the low
+// half-word, back count in the high one); every lane then writes its records
+// without further cross-lane work.  Absent records are written to the lane's
+// own slot past the ring (kL1Cap + lane), so the writes need no branches.
+// Order: lane-major, then k; front records are placed downwards from the old
+// head, so the first one pushed ends next to it.
 template <int K>
-__device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
-                        const uint64_t (&mask)[K], uint32_t spill, uint32_t fifo,
-                        const L2Queue &q, WaveStats &st) {
+__device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K], uint32_t bits,
+                        uint32_t spill, uint32_t fifo, const L2Queue &q, WaveStats &st) {
     l1_uniform(l1);
-    uint32_t total = 0;
+    const uint32_t lane = lane_id();
+    const uint32_t cnt = (uint32_t)__builtin_popcount(bits);
     // Pivot: the head's distance, or on an empty deque the first pushed
-    // record's (first register with a record, lowest lane).
-    int first_k = K;
-#pragma unroll
-    for (int k = K - 1; k >= 0; --k) {
-        if (mask[k]) first_k = k;
-        total += popc(mask[k]);
-    }
-    if (total == 0) return true;
+    // record's (lowest lane with a record, its lowest k).
     uint32_t pivot = 0;   // stays 0 in FIFO mode: no record is below it
+    const uint64_t any = ballot(bits != 0);
+    if (any == 0) return true;
     if (fifo) {
     } else if (l1.size) {
         pivot = uniform(rec_dist(ring[l1.head]));
     } else {
+        uint32_t fd = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (k == first_k)
-                pivot = read_lane(rec_dist(rec[k]), (uint32_t)__builtin_ctzll(mask[k]));
+        for (int k = K - 1; k >= 0; --k)
+            if ((bits >> k) & 1) fd = rec_dist(rec[k]);
+        pivot = read_lane(fd, (uint32_t)__builtin_ctzll(any));
     }
-    const uint32_t lane = lane_id();
-    const uint64_t lt = lanemask_lt();
-    uint32_t nf = 0, nb = 0;
+    uint32_t fbits = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (((bits >> k) & 1) && rec_dist(rec[k]) < pivot) fbits |= 1u << k;
+    const uint32_t nf = (uint32_t)__builtin_popcount(fbits);
+    uint32_t tot = 0;
+    const uint32_t ex = wave_excl_scan(nf | ((cnt - nf) << 16), tot);
+    const uint32_t total_f = tot & 0xFFFFu, total = total_f + (tot >> 16);
+    uint32_t pf = l1.head - 1 - (ex & 0xFFFFu);
+    uint32_t pb = l1.head + l1.size + (ex >> 16);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const bool valid = (mask[k] >> lane) & 1;
-        const bool front = valid && rec_dist(rec[k]) < pivot;
-        const uint64_t fm = ballot(front);
-        const uint64_t bm = mask[k] & ~fm;
-        uint32_t p = 0;
-        if (front) p = l1.head - 1 - (nf + popc(fm & lt));
-        else if (valid) p = l1.head + l1.size + nb + popc(bm & lt);
-        if (valid) ring[p & kL1Mask] = rec[k];
-        nf += popc(fm);
-        nb += popc(bm);
+        const bool v = (bits >> k) & 1;
+        const bool f = (fbits >> k) & 1;
+        const uint32_t p = v ? ((f ? pf : pb) & kL1Mask) : kL1Cap + lane;
+        ring[p] = rec[k];
+        pf -= f ? 1u : 0u;
+        pb += (v && !f) ? 1u : 0u;
     }
     wave_sync();
-    l1.head = (l1.head - nf) & kL1Mask;
+    l1.head = (l1.head - total_f) & kL1Mask;
     l1.size += total;
     if (l1.size > spill) return l1_spill_all(ring, l1, q, st);
     return true;

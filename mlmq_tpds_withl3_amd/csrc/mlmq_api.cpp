@@ -330,6 +330,7 @@ void mlmq_params_default(mlmq_params *p) {
     p->batch_waves = 8;
     p->use_graph = 1;
     p->hub_split = 0;
+    p->stale_check = 2;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -506,6 +507,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
     if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
     if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
+    if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -554,6 +556,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.blocks = c->blocks;
         L.hubs = c->hubs;
         L.split = p.hub_split != 0;
+        L.stale_check = (uint32_t)p.stale_check;
         L.dist = c->dist;
         L.slots = c->slots;
         L.ctl = c->ctl;
@@ -663,6 +666,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
     if (p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
         return fail(c, MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
+    if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
     if (p.work_blocks < 0 || p.bucket_capacity < 0) return fail(c, MLMQ_E_INVALID, "negative size parameter");
     for (int32_t i = 0; i < nq; ++i)
         if (sources[i] < 0 || sources[i] >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
@@ -736,6 +740,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.blocks = c->blocks;
         L.hubs = c->hubs;
         L.split = p.hub_split != 0;
+        L.stale_check = (uint32_t)p.stale_check;
         L.dist = nullptr;
         L.slots = nullptr;
         L.ctl = nullptr;
@@ -776,7 +781,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             L.timeout_ticks, (uint64_t)(uintptr_t)c->b_dist, (uint64_t)(uintptr_t)c->b_slots,
             (uint64_t)(uintptr_t)c->b_ctl, (uint64_t)(uintptr_t)c->b_sources,
             (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks,
-            (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split};
+            (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

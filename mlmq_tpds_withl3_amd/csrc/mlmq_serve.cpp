@@ -481,6 +481,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.blocks = g.v.blocks;
         L.hubs = g.v.hubs;
         L.split = g.p.hub_split != 0;
+        L.stale_check = 2;
         L.n = (uint32_t)g.v.n;
         L.nbuckets = (uint32_t)g.p.buckets;
         L.window = (uint32_t)g.p.bucket_window;

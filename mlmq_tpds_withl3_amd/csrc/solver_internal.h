@@ -59,6 +59,7 @@ struct SolverLaunch {
     bool slf;       // L1 in LDS (SLF deque or FIFO); false: L2 only
     bool fifo;      // with slf: L1 is a FIFO (queue mode L1V_L2DQ)
     bool split;     // split hub edge lists into chunk records
+    uint32_t stale_check;   // 2 coherent, 1 cached, 0 none
     bool profile;   // per-phase clock build of the kernel
 };
 

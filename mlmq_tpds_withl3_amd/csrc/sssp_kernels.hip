@@ -38,6 +38,7 @@ struct SolveArgs {
     uint32_t donate_min;    // donate to L2 while waves are idle and L1 holds more (0 = off)
     uint32_t fifo;          // 1: L1 is a FIFO (queue mode L1V_L2DQ), 0: SLF deque
     uint32_t split;         // 1: split hub edge lists into chunk records
+    uint32_t stale_check;   // 2: coherent dist load, 1: cached load, 0: none
     uint64_t timeout_ticks;
 };
 
@@ -64,19 +65,6 @@ __device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
     case 2: return q.z;
     default: return q.w;
     }
-}
-
-template <bool kSLF>
-__device__ __forceinline__ bool push_one(uint64_t *ring, L1State &l1, uint64_t rec, bool improved,
-                                         const SolveArgs &a, WaveStats &st) {
-    const uint64_t im = ballot(improved);
-    if (!im) return true;
-    if (kSLF) {
-        const uint64_t r[1] = {rec};
-        const uint64_t m[1] = {im};
-        return l1_push<1>(ring, l1, r, m, a.spill, a.fifo, a.q, st);
-    }
-    return l2_push(a.q, rec, improved, st);
 }
 
 // Heavy records (more out-edges than a hop block holds): edge slots spread
@@ -139,24 +127,24 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         }
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pv[j]));
-        uint64_t out[kHeavyUnroll], im[kHeavyUnroll];
-        uint32_t relax = 0, improved = 0;
+        uint64_t out[kHeavyUnroll];
+        uint32_t bits = 0, nact = 0;
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) {
-            im[j] = ballot(act[j] && nd[j] < pv[j]);
+            if (act[j] && nd[j] < pv[j]) bits |= 1u << j;
+            nact += act[j] ? 1u : 0u;
             out[j] = make_rec(c[j], nd[j]);
-            relax += popc(ballot(act[j]));
-            improved += popc(im[j]);
         }
-        st.v[kStatRelax] += relax;
-        st.v[kStatImprove] += improved;
-        if (improved) {
+        const uint32_t sums = wave_sum(nact | ((uint32_t)__builtin_popcount(bits) << 16));
+        st.v[kStatRelax] += sums & 0xFFFFu;
+        st.v[kStatImprove] += sums >> 16;
+        if (sums >> 16) {
             if (kSLF) {
-                ok = l1_push<kHeavyUnroll>(ring, l1, out, im, a.spill, a.fifo, a.q, st);
+                ok = l1_push<kHeavyUnroll>(ring, l1, out, bits, a.spill, a.fifo, a.q, st);
             } else {
 #pragma unroll
                 for (int j = 0; j < kHeavyUnroll; ++j)
-                    if (ok && im[j]) ok = l2_push(a.q, out[j], (im[j] >> lane) & 1, st);
+                    if (ok) ok = l2_push(a.q, out[j], (bits >> j) & 1, st);
             }
         }
     }
@@ -218,10 +206,15 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
             v = a.hubs[(v & ~kChunkFlag) >> kChunkBits].x;
         }
     }
-    uint32_t cur = 0;
+    // Stale filter (reference :174, process on <=).  Distances only fall, so
+    // a value read through this XCD's L2 (a.stale_check == 1) is never below
+    // the coherent one: the filter then lets some stale records through but
+    // never drops a live one.  0 = no filter (every record is expanded).
+    uint32_t cur = kUnreached;
     HopBlock b = empty_block();
     if (valid) {
-        cur = ld_relaxed(a.dist + v);
+        if (a.stale_check == 2) cur = ld_relaxed(a.dist + v);
+        else if (a.stale_check == 1) cur = a.dist[v];
         b = load_block(a.blocks, v);
     }
     // idle-wave count for work donation, in the same round trip
@@ -275,28 +268,27 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     // for the atomics one at a time.
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pv[k]));
-    uint64_t out[kInlineEdges], im[kInlineEdges];
-    uint32_t relax = 0, improved = 0;
+    uint64_t out[kInlineEdges];
+    uint32_t bits = 0;   // bit k: edge k lowered its destination's distance
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
-        const bool on = (uint32_t)k < ideg;
-        im[k] = ballot(on && nd[k] < pv[k]);
+        if ((uint32_t)k < ideg && nd[k] < pv[k]) bits |= 1u << k;
         out[k] = make_rec(c[k], nd[k]);
-        relax += popc(ballot(on));
-        improved += popc(im[k]);
     }
-    st.v[kStatRelax] += relax;
-    st.v[kStatImprove] += improved;
+    // relaxations and improvements of the batch in one wave reduction
+    const uint32_t sums = wave_sum(ideg | ((uint32_t)__builtin_popcount(bits) << 16));
+    st.v[kStatRelax] += sums & 0xFFFFu;
+    st.v[kStatImprove] += sums >> 16;
     clk.lap(st, kClkAtomic);
 
     bool ok = true;
-    if (improved) {
+    if (sums >> 16) {
         if (kSLF) {
-            ok = l1_push<kInlineEdges>(ring, l1, out, im, a.spill, a.fifo, a.q, st);
+            ok = l1_push<kInlineEdges>(ring, l1, out, bits, a.spill, a.fifo, a.q, st);
         } else {
 #pragma unroll
             for (int k = 0; k < (int)kInlineEdges; ++k)
-                if (ok && im[k]) ok = l2_push(a.q, out[k], (im[k] >> lane) & 1, st);
+                if (ok) ok = l2_push(a.q, out[k], (bits >> k) & 1, st);
         }
         clk.lap(st, kClkFlush);
     }
@@ -413,7 +405,7 @@ __device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
 
 template <int kWaves, bool kSLF, bool kProf>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
-    __shared__ uint64_t s_ring[kWaves][kL1Cap];
+    __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
 
@@ -459,7 +451,7 @@ struct BatchArgs {
 
 template <int kWaves, bool kProf>
 __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
-    __shared__ uint64_t s_ring[kWaves][kL1Cap];
+    __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
     __shared__ uint32_t s_query;
@@ -556,7 +548,7 @@ __global__ __launch_bounds__(256) void serve_reset_kernel(ServeRequest *reqs, ui
 template <int kWaves>
 __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base, ServeRequest *reqs,
                                                                  uint32_t wg_per_req) {
-    __shared__ uint64_t s_ring[kWaves][kL1Cap];
+    __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
 
@@ -787,6 +779,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.donate_min = L.donate_min;
     a.fifo = L.fifo ? 1u : 0u;
     a.split = L.hubs && L.split ? 1u : 0u;
+    a.stale_check = L.stale_check;
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }

@@ -76,6 +76,14 @@ def test_hub_batch(solver, split):
     check_batch(solver, row, col, w, sources, work_blocks=3, hub_split=split, delta=300)
 
 
+@pytest.mark.parametrize("stale", [0, 1])
+def test_stale_modes_batch(solver, stale):
+    say(f"stale_check={stale} batch")
+    row, col, w = grid_csr(70, 80, 5)
+    solver.load_csr(row, col, w)
+    check_batch(solver, row, col, w, [0, 5599, 123, 4000], work_blocks=2, stale_check=stale)
+
+
 def test_graph_replays_interleaved(solver):
     say("graph replays with eager work in between")
     row, col, w = grid_csr(80, 90, 21)

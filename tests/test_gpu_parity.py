@@ -78,6 +78,8 @@ def test_random_graphs(solver, seed, queue):
     dict(queue="L1N_L2DQ", delta=2000),
     dict(queue="L1V_L2DQ", delta=2000),
     dict(queue="L1V_L2DQ", l1_spill=8, l1_pop=16),               # FIFO spilling often
+    dict(stale_check=1, delta=2000),                             # stale filter through the L2
+    dict(stale_check=0, delta=2000),                             # no stale filter
 ])
 def test_grid_param_corners(solver, cfg):
     say(f"grid 120x150 {cfg}")
@@ -131,7 +133,8 @@ def test_road264k(solver):
     say("road264k generate")
     g, src, spec = make_workload("road264k")
     row, col, w = g.arrays()
-    for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ")):
+    for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ"),
+                dict(stale_check=1), dict(stale_check=0)):
         say(f"road264k {cfg}")
         t, work, reached = solve_check(solver, row, col, w, src, **cfg)
         assert t["relaxations"] >= work
