@@ -93,7 +93,14 @@ typedef struct mlmq_params {
                                  2 (default) coherent load of dist[v], 1 load through the L2
                                  (may miss a drop made on another XCD), 0 none. Results are
                                  exact in every mode; only the work differs */
-    int32_t  reserved[3];
+    int32_t  prefetch;        /* 1 (default): while its atomicMin is in flight, touch each
+                                 relaxed child's hop block so a later pop finds it in L2 */
+    int32_t  l1_blocks;       /* mlmq_solve with an L1: 1 = each L1 record carries its hop block
+                                 in LDS, loaded while the record's atomicMin was in flight, so
+                                 popping it needs no global load; 0 (default) = records only
+                                 (measured faster on MI355X: the extra loads and LDS block
+                                 traffic cost more than the round trip they save) */
+    int32_t  reserved[1];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */

@@ -17,6 +17,7 @@
 // scalar registers, not LDS words updated by lane 0 as in the reference.
 #pragma once
 
+#include "hop_block.h"
 #include "l2_delta_queue.h"
 
 namespace mlmq {
@@ -24,6 +25,11 @@ namespace mlmq {
 constexpr uint32_t kL1Cap = 512;
 constexpr uint32_t kL1Mask = kL1Cap - 1;
 constexpr uint32_t kL1Alloc = kL1Cap + 64;   // ring + one discard slot per lane
+// Block-carrying L1 (single-query kernel): ring slot i is paired with block
+// slot i, which holds the hop block of the record in slot i, loaded while
+// that record was being relaxed into existence, or kColdMark in word 0 when
+// the block is not there (the record then takes the global round trip).
+constexpr uint32_t kColdMark = 0xFFFFFFFFu;
 
 struct L1State {
     uint32_t head;
@@ -44,6 +50,33 @@ __device__ __forceinline__ uint32_t l1_pop(const uint64_t *ring, L1State &l1, ui
     const uint32_t k = uniform(l1.size < maxn ? l1.size : maxn);
     if (lane_id() < k) rec = ring[(l1.head + lane_id()) & kL1Mask];
     wave_sync();
+    l1.head = (l1.head + k) & kL1Mask;
+    l1.size -= k;
+    return k;
+}
+
+// Pop for the block-carrying L1: like l1_pop, and each popped lane also gets
+// the block slot of its record.  When the head record is hot (its block is
+// in LDS) only the leading run of hot records is taken, so the batch needs
+// no global round trip for them.
+__device__ __forceinline__ uint32_t l1_pop_hot(const uint64_t *ring, const HopBlock *blk,
+                                               L1State &l1, uint32_t maxn, uint64_t &rec,
+                                               bool &hot, uint32_t (&hw)[16]) {
+    l1_uniform(l1);
+    const uint32_t lane = lane_id();
+    uint32_t k = uniform(l1.size < maxn ? l1.size : maxn);
+    // Every lane reads (slots past the popped ones are in the ring too), so
+    // the block stays in registers with no branch around it.
+    const uint32_t pos = (l1.head + lane) & kL1Mask;
+    const uint64_t r = ring[pos];
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(blk + pos);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) hw[i] = src[i];
+    if (lane < k) rec = r;
+    hot = lane < k && hw[0] != kColdMark;
+    const uint64_t hm = ballot(hot);
+    if (hm & 1) k = uniform((uint32_t)__builtin_ctzll(~hm));   // leading hot run (<= k)
+    hot = hot && lane < k;
     l1.head = (l1.head + k) & kL1Mask;
     l1.size -= k;
     return k;
@@ -90,9 +123,14 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
 // own slot past the ring (kL1Cap + lane), so the writes need no branches.
 // Order: lane-major, then k; front records are placed downwards from the old
 // head, so the first one pushed ends next to it.
-template <int K>
-__device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K], uint32_t bits,
-                        uint32_t spill, uint32_t fifo, const L2Queue &q, WaveStats &st) {
+//
+// kBlk selects what goes into the block slots of a block-carrying L1:
+// 0 = no block slots, 1 = the records' blocks `cb[k]`, 2 = kColdMark.
+template <int K, int kBlk = 0>
+__device__ __forceinline__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
+                                        uint32_t bits, uint32_t spill, uint32_t fifo,
+                                        const L2Queue &q, WaveStats &st, HopBlock *blk = nullptr,
+                                        const HopBlock *cb = nullptr) {
     l1_uniform(l1);
     const uint32_t lane = lane_id();
     const uint32_t cnt = (uint32_t)__builtin_popcount(bits);
@@ -127,6 +165,8 @@ __device__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K], u
         const bool f = (fbits >> k) & 1;
         const uint32_t p = v ? ((f ? pf : pb) & kL1Mask) : kL1Cap + lane;
         ring[p] = rec[k];
+        if (kBlk == 1 && v) blk[p] = cb[k];
+        if (kBlk == 2 && v) reinterpret_cast<uint32_t *>(blk + p)[0] = kColdMark;
         pf -= f ? 1u : 0u;
         pb += (v && !f) ? 1u : 0u;
     }

@@ -331,6 +331,8 @@ void mlmq_params_default(mlmq_params *p) {
     p->use_graph = 1;
     p->hub_split = 0;
     p->stale_check = 2;
+    p->prefetch = 1;
+    p->l1_blocks = 0;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -481,7 +483,7 @@ int mlmq_device_shape(mlmq_ctx *c, int32_t *blocks_per_cu_max, int32_t *cus, int
     if (!c) return MLMQ_E_INVALID;
     HIP_TRY(c, hipSetDevice(c->device));
     int b = 0;
-    HIP_TRY(c, mlmq::solver_occupancy(true, &b));
+    HIP_TRY(c, mlmq::solver_occupancy(true, false, &b));
     if (blocks_per_cu_max) *blocks_per_cu_max = b;
     if (cus) *cus = c->cus;
     if (threads) *threads = mlmq::kWavesPerBlock * mlmq::kWave;
@@ -508,12 +510,14 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
     if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
     if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
+    if (p.l1_blocks != 0 && p.l1_blocks != 1) return fail(c, MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
 
     int occ = 0;
-    HIP_TRY(c, mlmq::solver_occupancy(slf, &occ));
+    const bool hot = slf && p.l1_blocks != 0;
+    HIP_TRY(c, mlmq::solver_occupancy(slf, hot, &occ));
     if (occ < 1) return fail(c, MLMQ_E_HIP, "solver kernel cannot be resident");
     int blocks = p.work_blocks;
     if (blocks == 0 && p.blocks_per_cu > 0) {
@@ -557,6 +561,8 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.hubs = c->hubs;
         L.split = p.hub_split != 0;
         L.stale_check = (uint32_t)p.stale_check;
+        L.prefetch = p.prefetch ? 1u : 0u;
+        L.hot = hot;
         L.dist = c->dist;
         L.slots = c->slots;
         L.ctl = c->ctl;
@@ -741,6 +747,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.hubs = c->hubs;
         L.split = p.hub_split != 0;
         L.stale_check = (uint32_t)p.stale_check;
+        L.prefetch = p.prefetch ? 1u : 0u;
+        L.hot = false;
         L.dist = nullptr;
         L.slots = nullptr;
         L.ctl = nullptr;
@@ -781,7 +789,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             L.timeout_ticks, (uint64_t)(uintptr_t)c->b_dist, (uint64_t)(uintptr_t)c->b_slots,
             (uint64_t)(uintptr_t)c->b_ctl, (uint64_t)(uintptr_t)c->b_sources,
             (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks,
-            (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check};
+            (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check,
+            (uint64_t)L.prefetch};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

@@ -482,6 +482,8 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.hubs = g.v.hubs;
         L.split = g.p.hub_split != 0;
         L.stale_check = 2;
+        L.prefetch = 1;
+        L.hot = false;
         L.n = (uint32_t)g.v.n;
         L.nbuckets = (uint32_t)g.p.buckets;
         L.window = (uint32_t)g.p.bucket_window;

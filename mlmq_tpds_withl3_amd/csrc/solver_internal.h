@@ -60,6 +60,8 @@ struct SolverLaunch {
     bool fifo;      // with slf: L1 is a FIFO (queue mode L1V_L2DQ)
     bool split;     // split hub edge lists into chunk records
     uint32_t stale_check;   // 2 coherent, 1 cached, 0 none
+    uint32_t prefetch;      // 1: prefetch relaxed children's hop blocks
+    bool hot;               // with slf: block-carrying L1 (single-query kernel only)
     bool profile;   // per-phase clock build of the kernel
 };
 
@@ -113,7 +115,7 @@ hipError_t launch_patch_hubs(const uint2 *hubs, uint32_t nh, uint4 *blocks, hipS
 hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
                                uint32_t n, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);
-hipError_t solver_occupancy(bool slf, int *blocks_per_cu);
+hipError_t solver_occupancy(bool slf, bool hot, int *blocks_per_cu);
 hipError_t launch_solver(const SolverLaunch &L, hipStream_t s);
 
 }  // namespace mlmq

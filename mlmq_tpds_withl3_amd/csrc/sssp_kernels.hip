@@ -39,6 +39,7 @@ struct SolveArgs {
     uint32_t fifo;          // 1: L1 is a FIFO (queue mode L1V_L2DQ), 0: SLF deque
     uint32_t split;         // 1: split hub edge lists into chunk records
     uint32_t stale_check;   // 2: coherent dist load, 1: cached load, 0: none
+    uint32_t prefetch;      // 1: touch each relaxed child's hop block during the atomics
     uint64_t timeout_ticks;
 };
 
@@ -74,10 +75,10 @@ __device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
 // column/weight loads, then all their atomicMins, are in flight together.
 constexpr int kHeavyUnroll = 4;
 
-template <bool kSLF>
+template <bool kSLF, int kBlk>
 __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
-                            uint32_t hdeg, uint64_t *ring, L1State &l1, uint32_t *map,
-                            WaveStats &st) {
+                            uint32_t hdeg, uint64_t *ring, HopBlock *blk, L1State &l1,
+                            uint32_t *map, WaveStats &st) {
     const uint32_t lane = lane_id();
     const uint32_t deg = heavy ? hdeg : 0u;
     uint32_t total = 0;
@@ -140,7 +141,8 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         st.v[kStatImprove] += sums >> 16;
         if (sums >> 16) {
             if (kSLF) {
-                ok = l1_push<kHeavyUnroll>(ring, l1, out, bits, a.spill, a.fifo, a.q, st);
+                ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill, a.fifo,
+                                                         a.q, st, blk);
             } else {
 #pragma unroll
                 for (int j = 0; j < kHeavyUnroll; ++j)
@@ -188,10 +190,11 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 //      back (up to 7 per lane) before any result is used.
 // A (dst, nd) record is pushed only if its atomicMin strictly lowered dst's
 // distance (:211-243).  The batch's pushes go to L1 in one pass.
-template <bool kSLF, bool kProf>
-__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, uint64_t *ring,
-                              L1State &l1, uint32_t *map, uint32_t (&pf)[kInlineEdges],
-                              uint32_t &hungry, WaveStats &st, PhaseClock<kProf> &clk) {
+template <bool kSLF, bool kProf, bool kHot>
+__device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
+                              const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
+                              uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &hungry,
+                              WaveStats &st, PhaseClock<kProf> &clk) {
     const uint32_t lane = lane_id();
     const bool valid = lane < nin;
     uint32_t v = rec_vertex(rec);
@@ -210,16 +213,28 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     // a value read through this XCD's L2 (a.stale_check == 1) is never below
     // the coherent one: the filter then lets some stale records through but
     // never drops a live one.  0 = no filter (every record is expanded).
+    // Hot lanes (block-carrying L1) already hold their block: no round trip;
+    // their stale check is made after the atomics (see below).
     uint32_t cur = kUnreached;
     HopBlock b = empty_block();
-    if (valid) {
+    if (valid && !hot) {
         if (a.stale_check == 2) cur = ld_relaxed(a.dist + v);
         else if (a.stale_check == 1) cur = a.dist[v];
         b = load_block(a.blocks, v);
     }
+    if (kHot) {
+        // word-wise select of register values (keeps both sources out of memory)
+        uint32_t *bw = reinterpret_cast<uint32_t *>(&b);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint32_t h = hw[i];
+            asm volatile("" : "+v"(h));
+            bw[i] = hot ? h : bw[i];
+        }
+    }
     // idle-wave count for work donation, in the same round trip
     hungry = a.donate_min ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
-    const bool keep = valid && d <= cur;
+    const bool keep = valid && (hot || d <= cur);
     st.v[kStatStale] += popc(ballot(valid && !keep));
     uint32_t deg = keep ? b.deg() : 0u;
     uint32_t hstart = b.start();
@@ -253,27 +268,49 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     // atomic).  Unpredicated -- lanes without an edge touch vertex 0's
     // block -- so there is no branch and no wait; the loaded words are
     // consumed one batch later.
+    // Block-carrying L1: load every relaxed child's whole block in this
+    // round trip; the improved children take it into L1 with them, and the
+    // batch that pops them needs no global load.
+    HopBlock cb[kHot ? kInlineEdges : 1];
+    if (kHot) {
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) {
-        const uint32_t t = (uint32_t)k < ideg ? c[k] : 0u;
-        pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
+        for (int k = 0; k < (kHot ? (int)kInlineEdges : 0); ++k)
+            cb[k] = load_block(a.blocks, (uint32_t)k < ideg ? c[k] : 0u);
+    } else if (a.prefetch) {
+#pragma unroll
+        for (int k = 0; k < (int)kInlineEdges; ++k) {
+            const uint32_t t = (uint32_t)k < ideg ? c[k] : 0u;
+            pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
+        }
     }
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         pv[k] = 0;
         if ((uint32_t)k < ideg) pv[k] = fetch_min(a.dist + c[k], nd[k]);
     }
+    // Hot lanes' stale check, in the same round trip.  A record whose vertex
+    // has since dropped below d has a fresher record queued; that one relaxes
+    // every child to a strictly lower distance and pushes it, so the children
+    // this stale expansion improved need not be pushed.
+    uint32_t cur2 = 0;
+    if (kHot && hot && a.stale_check) cur2 = ld_relaxed(a.dist + v);
     // Keep every result opaque until all atomics are issued; otherwise the
     // compiler sinks each comparison into its predicated block and waits
     // for the atomics one at a time.
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pv[k]));
+    if (kHot) asm volatile("" : "+v"(cur2));
     uint64_t out[kInlineEdges];
     uint32_t bits = 0;   // bit k: edge k lowered its destination's distance
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         if ((uint32_t)k < ideg && nd[k] < pv[k]) bits |= 1u << k;
         out[k] = make_rec(c[k], nd[k]);
+    }
+    if (kHot) {
+        const bool late_stale = hot && a.stale_check && cur2 < d;
+        st.v[kStatStale] += popc(ballot(late_stale));
+        if (late_stale) bits = 0;
     }
     // relaxations and improvements of the batch in one wave reduction
     const uint32_t sums = wave_sum(ideg | ((uint32_t)__builtin_popcount(bits) << 16));
@@ -284,7 +321,8 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
     bool ok = true;
     if (sums >> 16) {
         if (kSLF) {
-            ok = l1_push<kInlineEdges>(ring, l1, out, bits, a.spill, a.fifo, a.q, st);
+            ok = l1_push<kInlineEdges, kHot ? 1 : 0>(ring, l1, out, bits, a.spill, a.fifo, a.q,
+                                                     st, blk, cb);
         } else {
 #pragma unroll
             for (int k = 0; k < (int)kInlineEdges; ++k)
@@ -293,7 +331,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
         clk.lap(st, kClkFlush);
     }
     if (ok && ballot(heavy)) {
-        ok = relax_heavy<kSLF>(a, heavy, d, hstart, deg, ring, l1, map, st);
+        ok = relax_heavy<kSLF, kHot ? 1 : 0>(a, heavy, d, hstart, deg, ring, blk, l1, map, st);
         clk.lap(st, kClkEdges);
     }
     return ok;
@@ -303,10 +341,10 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, ui
 // loop, SS/SSSP/sssp_run.cu:507-584).  Returns when the query's exit flag is
 // seen, after an error, or at the deadline.  `leader` waves also run the
 // termination check while idle.
-template <bool kSLF, bool kProf>
+template <bool kSLF, bool kProf, bool kHot = false>
 __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
                            uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
-                           uint64_t deadline) {
+                           uint64_t deadline, HopBlock *blk = nullptr) {
     const uint32_t lane = lane_id();
     const L2Queue &q = a.q;
     L1State l1{0u, 0u};
@@ -323,11 +361,19 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
     for (;;) {
         uint64_t rec = kEmpty;
         uint32_t nin = 0;
-        if (kSLF) {
+        bool hot = false;
+        uint32_t hw[16];
+        if (kHot) {
+            nin = l1_pop_hot(ring, blk, l1, a.pop_max, rec, hot, hw);
+            clk.lap(st, kClkL1Pop);
+        } else if (kSLF) {
             nin = l1_pop(ring, l1, a.pop_max, rec);
             clk.lap(st, kClkL1Pop);
         }
-        if (nin < a.min_read) {  // here L1 is empty
+        // Top up from L2 below min_read.  A block-carrying L1 may hold back
+        // cold records behind a hot run; it tops up only once it is empty, so
+        // a hot batch stays free of global loads.
+        if (nin < a.min_read && (!kHot || l1.size == 0)) {
             const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, kWave - nin, st);
             if (got) {
                 credit += got;
@@ -364,7 +410,9 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf>(a, rec, nin, ring, l1, map, pf, hungry, st, clk)) break;
+        if (!process_batch<kSLF, kProf, kHot>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
+                                              st, clk))
+            break;
         if (kSLF && hungry && l1.size > a.donate_min) {
             // (uniform: hungry is a scalar load result)
             if (!l1_donate(ring, l1, uniform(l1.size) / 2, q, st)) break;
@@ -403,11 +451,15 @@ __device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
     }
 }
 
-template <int kWaves, bool kSLF, bool kProf>
+// kHot: block-carrying L1 (each wave's ring paired with 512 hop-block slots in
+// LDS, 32 KiB per wave; about 150 KiB per 4-wave workgroup, so one
+// workgroup per CU).
+template <int kWaves, bool kSLF, bool kProf, bool kHot>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
     __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
+    __shared__ uint4 s_blk[kHot ? kWaves : 1][kHot ? 4 * kL1Cap : 1];
 
     const uint32_t w = threadIdx.x / kWave;
     const uint32_t gwave = blockIdx.x * kWaves + w;
@@ -422,7 +474,8 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
         for (int i = 0; i < 16; ++i) clk.lap(st, kClkLaps);
     }
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<kSLF, kProf>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline);
+    run_worker<kSLF, kProf, kHot>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline,
+                                  kHot ? reinterpret_cast<HopBlock *>(s_blk[w]) : nullptr);
     if (kProf) st.v[kClkTotal] = clock64() - t_begin;
     flush_stats(a.q.ctl, st);
 }
@@ -734,13 +787,15 @@ hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, h
     return hipGetLastError();
 }
 
-static const void *solver_fn(bool slf) {
-    return slf ? reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true, false>)
-               : reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, false, false>);
+static const void *solver_fn(bool slf, bool hot) {
+    if (slf && hot)
+        return reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true, false, true>);
+    return slf ? reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true, false, false>)
+               : reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, false, false, false>);
 }
 
-hipError_t solver_occupancy(bool slf, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, solver_fn(slf),
+hipError_t solver_occupancy(bool slf, bool hot, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, solver_fn(slf, hot),
                                                         kWavesPerBlock * kWave, 0);
 }
 
@@ -780,6 +835,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.fifo = L.fifo ? 1u : 0u;
     a.split = L.hubs && L.split ? 1u : 0u;
     a.stale_check = L.stale_check;
+    a.prefetch = L.prefetch;
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }
@@ -851,14 +907,19 @@ hipError_t launch_serve_verify(const ServeLaunch &S, uint32_t n, hipStream_t s) 
 hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     const SolveArgs a = make_args(L);
     const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
-    if (L.slf && !L.profile)
-        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false>), grid, block, 0, s, a);
+    const bool hot = L.slf && L.hot;
+    if (L.slf && hot && !L.profile)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, true>), grid, block, 0, s, a);
+    else if (L.slf && hot)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, true>), grid, block, 0, s, a);
+    else if (L.slf && !L.profile)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, false>), grid, block, 0, s, a);
     else if (L.slf)
-        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, false>), grid, block, 0, s, a);
     else if (!L.profile)
-        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, false>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, false, false>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, true>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, true, false>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 

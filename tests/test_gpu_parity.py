@@ -63,7 +63,8 @@ def test_random_graphs(solver, seed, queue):
     n = 200 + 700 * seed
     row, col, w = random_csr(n, 4 * n, seed, wmax=1000, zero_frac=0.1)
     for src in (0, n - 1):
-        solve_check(solver, row, col, w, src, queue=queue, delta=300)
+        for blocks in ((1, 0) if queue != "L1N_L2DQ" else (1,)):
+            solve_check(solver, row, col, w, src, queue=queue, delta=300, l1_blocks=blocks)
 
 
 @pytest.mark.parametrize("cfg", [
@@ -80,6 +81,10 @@ def test_random_graphs(solver, seed, queue):
     dict(queue="L1V_L2DQ", l1_spill=8, l1_pop=16),               # FIFO spilling often
     dict(stale_check=1, delta=2000),                             # stale filter through the L2
     dict(stale_check=0, delta=2000),                             # no stale filter
+    dict(prefetch=0, delta=2000),
+    dict(l1_blocks=1, delta=2000),                               # block-carrying L1
+    dict(l1_blocks=1, queue="L1V_L2DQ"),
+    dict(l1_blocks=1, stale_check=0, l1_pop=8, min_read=8),      # hot pops, no stale filter
 ])
 def test_grid_param_corners(solver, cfg):
     say(f"grid 120x150 {cfg}")
@@ -98,6 +103,7 @@ HUBS = [70000, 9000, 4097, 2049, 2048, 1500]   # degrees 70002 .. 1504: five hub
     dict(work_blocks=1),                         # one workgroup takes every chunk itself
     dict(delta=100, l2_batch=64, min_read=64),
     dict(l1_spill=4, l1_pop=8),
+    dict(l1_blocks=1),
 ])
 def test_hub_split(solver, cfg):
     say(f"hubs {cfg}")
@@ -134,7 +140,7 @@ def test_road264k(solver):
     g, src, spec = make_workload("road264k")
     row, col, w = g.arrays()
     for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ"),
-                dict(stale_check=1), dict(stale_check=0)):
+                dict(stale_check=1), dict(stale_check=0), dict(l1_blocks=1)):
         say(f"road264k {cfg}")
         t, work, reached = solve_check(solver, row, col, w, src, **cfg)
         assert t["relaxations"] >= work
