@@ -144,6 +144,8 @@ def main():
         delta = best["delta"]
         args.l1_pop, args.min_read, args.work_blocks = best["l1_pop"], best["min_read"], best["work_blocks"]
         args.l2_batch = best.get("l2_batch", args.l2_batch)
+        args.buckets = best.get("buckets", args.buckets)
+        args.window = best.get("bucket_window", args.window)
         params = make_params(delta)
         log(f"tuned over delta {list(spec['delta_sweep'])}: {best} {tuned['device_ms']:.3f} ms")
         if args.extended_sweep:

@@ -100,7 +100,8 @@ typedef struct mlmq_params {
                                  popping it needs no global load; 0 (default) = records only
                                  (measured faster on MI355X: the extra loads and LDS block
                                  traffic cost more than the round trip they save) */
-    int32_t  reserved[1];
+    int32_t  idle_backoff;    /* longest idle sleep between polls of an empty queue, in units of
+                                 256 cycles, 1..64; 0 = 16 */
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */

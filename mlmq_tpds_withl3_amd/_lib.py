@@ -61,7 +61,7 @@ class Params(ctypes.Structure):
         ("l1_donate", ctypes.c_int32), ("batch_waves", ctypes.c_int32),
         ("use_graph", ctypes.c_int32), ("hub_split", ctypes.c_int32),
         ("stale_check", ctypes.c_int32), ("prefetch", ctypes.c_int32),
-        ("l1_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
+        ("l1_blocks", ctypes.c_int32), ("idle_backoff", ctypes.c_int32),
     ]
 
 

@@ -34,8 +34,9 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None)
     from .solver import default_params
 
     if grid is None:
-        grid = [dict(l1_pop=pop, min_read=mr, l2_batch=lb, work_blocks=wb)
-                for wb in (64, 128)
+        grid = [dict(l1_pop=pop, min_read=mr, l2_batch=lb, work_blocks=wb, buckets=nb,
+                     bucket_window=min(4, nb))
+                for wb in (64, 128) for nb in (16, 4, 2)
                 for pop, mr, lb in ((32, 32, 8), (32, 32, 32), (32, 16, 16), (32, 8, 8), (64, 8, 8))]
     table = []
     for d in deltas:

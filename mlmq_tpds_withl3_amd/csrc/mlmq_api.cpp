@@ -184,7 +184,6 @@ int validate_csr(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const in
     return MLMQ_OK;
 }
 
-// Device-side hop blocks (64 B per vertex) from the device CSR.
 // Device-side hop blocks (64 B per vertex) from the device CSR, and the hub
 // table: vertices with more than kSplitDeg out-edges, highest degree first,
 // at most kMaxHubs of them, each with a chunk size that keeps its chunk count
@@ -333,6 +332,7 @@ void mlmq_params_default(mlmq_params *p) {
     p->stale_check = 2;
     p->prefetch = 1;
     p->l1_blocks = 0;
+    p->idle_backoff = 16;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -511,6 +511,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
     if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
     if (p.l1_blocks != 0 && p.l1_blocks != 1) return fail(c, MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
+    if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -574,7 +575,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.batch = (uint32_t)p.l2_batch;
         L.spin_limit = mlmq::kSpinLimit;
         L.min_read = (uint32_t)p.min_read;
-        L.max_sleep = 16;
+        L.max_sleep = p.idle_backoff > 0 ? (uint32_t)p.idle_backoff : 16u;
         L.pop_max = (uint32_t)p.l1_pop;
         L.spill = (uint32_t)p.l1_spill;
         L.donate_min = (uint32_t)p.l1_donate;
@@ -673,6 +674,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     if (p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
         return fail(c, MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
     if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
+    if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.work_blocks < 0 || p.bucket_capacity < 0) return fail(c, MLMQ_E_INVALID, "negative size parameter");
     for (int32_t i = 0; i < nq; ++i)
         if (sources[i] < 0 || sources[i] >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
@@ -760,7 +762,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.batch = (uint32_t)p.l2_batch;
         L.spin_limit = mlmq::kSpinLimit;
         L.min_read = (uint32_t)p.min_read;
-        L.max_sleep = 16;
+        L.max_sleep = p.idle_backoff > 0 ? (uint32_t)p.idle_backoff : 16u;
         L.pop_max = (uint32_t)p.l1_pop;
         L.spill = (uint32_t)p.l1_spill;
         L.donate_min = (uint32_t)p.l1_donate;
@@ -790,7 +792,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             (uint64_t)(uintptr_t)c->b_ctl, (uint64_t)(uintptr_t)c->b_sources,
             (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks,
             (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check,
-            (uint64_t)L.prefetch};
+            (uint64_t)L.prefetch, (uint64_t)L.max_sleep};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

@@ -46,8 +46,8 @@ def test_loader_binds_all():
     assert (p.queue_mode, p.delta, p.buckets, p.bucket_window, p.l2_batch, p.min_read) == \
         (19, 200000, 16, 4, 8, 8)
     assert (p.l1_pop, p.l1_spill, p.l1_donate, p.profile) == (32, 64, 0, 0)
-    assert (p.batch_waves, p.use_graph, p.hub_split, p.stale_check, p.prefetch, p.l1_blocks) == \
-        (8, 1, 0, 2, 1, 0)
+    assert (p.batch_waves, p.use_graph, p.hub_split, p.stale_check, p.prefetch, p.l1_blocks,
+            p.idle_backoff) == (8, 1, 0, 2, 1, 0, 16)
 
 
 def test_library_is_gfx950_and_independent_of_oracle():
@@ -82,7 +82,7 @@ def test_ctypes_layouts_match_c(tmp_path):
     C compiler gives it."""
     structs = {
         "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph", "hub_split",
-                                       "stale_check", "prefetch", "l1_blocks", "reserved"]),
+                                       "stale_check", "prefetch", "l1_blocks", "idle_backoff"]),
         "mlmq_timing": (_lib.Timing, ["donated", "hub_chunks"]),
         "mlmq_batch_timing": (_lib.BatchTiming, None),
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
