@@ -76,12 +76,17 @@ def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int
     best_s = score(cur)
     trail = [(dict(cur), best_s)]
     space = [("delta", _delta_grid(cur["delta"])), ("workgroups", [1, 2, 4, 8]),
-             ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64])]
+             ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64]), ("buckets", [16, 4, 2]),
+             ("min_read", [8, 16, 32])]
     for key, values in space:
         for v in values:
             if cur.get(key) == v:
                 continue
             cand = dict(cur, **{key: v})
+            if key == "buckets":
+                cand["bucket_window"] = min(4, v)
+            if key == "min_read":
+                cand["l2_batch"] = max(8, v)
             s = score(cand)
             trail.append((cand, s))
             if log:
@@ -114,14 +119,16 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
     best_t = score(cur)
     trail = [(dict(cur), best_t)]
     space = [("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
-             ("l1_pop", [16, 32, 64]), ("buckets", [8, 16, 32])]
+             ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32])]
     for key, values in space:
         for v in values:
             if cur.get(key) == v:
                 continue
             cand = dict(cur, **{key: v})
             if key == "buckets":
-                cand["bucket_window"] = min(cand.get("bucket_window", 4), v)
+                cand["bucket_window"] = min(4, v)
+            if key == "min_read":
+                cand["l2_batch"] = max(8, v)
             t = score(cand)
             trail.append((cand, t))
             if log:
