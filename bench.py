@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--min-read", type=int, default=8)
     ap.add_argument("--buckets", type=int, default=16)
     ap.add_argument("--window", type=int, default=4)
+    ap.add_argument("--light-split", type=int, default=0)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -123,7 +124,8 @@ def main():
                               blocks_per_cu=args.blocks_per_cu, l2_batch=args.l2_batch,
                               min_read=args.min_read, buckets=args.buckets,
                               bucket_window=args.window, l1_pop=args.l1_pop,
-                              l1_donate=args.l1_donate, timeout_s=30.0)
+                              l1_donate=args.l1_donate, light_split=args.light_split,
+                              timeout_s=30.0)
 
     solver = Solver(0)
     solver.load_csr(row, col, w)
@@ -137,7 +139,7 @@ def main():
         # The workload's delta sweep crossed with a small grid of L1 pop size,
         # L2 top-up threshold and grid size (untimed, best of two solves each).
         from mlmq_tpds_withl3_amd.autotune import tune_single
-        tuned = tune_single(solver, src, spec["delta_sweep"])
+        tuned = tune_single(solver, src, spec["delta_sweep"], grid=spec.get("tune_grid"))
         for prm, ms in tuned["table"]:
             sweep[prm["delta"]] = min(ms, sweep.get(prm["delta"], ms))
         best = tuned["params"]
@@ -146,10 +148,11 @@ def main():
         args.l2_batch = best.get("l2_batch", args.l2_batch)
         args.buckets = best.get("buckets", args.buckets)
         args.window = best.get("bucket_window", args.window)
+        args.light_split = best.get("light_split", args.light_split)
         params = make_params(delta)
         log(f"tuned over delta {list(spec['delta_sweep'])}: {best} {tuned['device_ms']:.3f} ms")
         if args.extended_sweep:
-            ext = tune_single(solver, src, (2_000_000, 20_000_000))
+            ext = tune_single(solver, src, (2_000_000, 20_000_000), grid=spec.get("tune_grid"))
             extended = {"params": ext["params"], "device_ms": round(ext["device_ms"], 4)}
             log(f"outside the configured sweep: {extended}")
     for i in range(args.warmup):
@@ -218,7 +221,7 @@ def main():
             "queue": args.queue, "delta": delta,
             "delta_sweep_device_ms": {str(k): round(v, 4) for k, v in sweep.items()},
             "outside_sweep_best": extended,
-            "buckets": args.buckets, "bucket_window": args.window,
+            "buckets": args.buckets, "bucket_window": args.window, "light_split": args.light_split,
             "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop, "l1_donate": args.l1_donate,
             "work_blocks": last["work_blocks"], "waves": last["waves"],
             "cus": shape["cus"], "blocks_per_cu_max": shape["blocks_per_cu_max"],
@@ -378,23 +381,28 @@ def run_serve(args):
             server.set_expected(gid, s_, r_)
         log(f"serve graph {gid} {name}: n={g.n} m={g.m} pool={len(pool)} "
             f"slot={server.slot_bytes(gid) / 2**20:.1f} MiB ({time.time() - t0:.1f}s)")
-        if args.tune:
-            t = tune_serving(server, gid, pool, base={"delta": spec["delta"]}, log=log)
-            tuned.append(t["params"])
-            log(f"serve graph {gid} tuned: {t['params']} {t['qps']:.1f} q/s")
-        else:
-            tuned.append({"delta": spec["delta"]})
-        # one-at-a-time comparator: plain solves of the pool sources
+        # one-at-a-time comparator: plain solves of the pool sources, tuned
+        # like the single-query mode; its parameters also seed the serving tuner
         with Solver(0) as sv:
             sv.load_csr(row, col, w)
             from mlmq_tpds_withl3_amd.autotune import tune_single
-            sp = default_params(timeout_s=60.0,
-                                **tune_single(sv, pool[0], spec["delta_sweep"])["params"])
+            single = tune_single(sv, pool[0], spec["delta_sweep"], grid=spec.get("tune_grid"))["params"]
+            sp = default_params(timeout_s=60.0, **single)
             sv.solve(pool[0], sp)
             t1 = time.perf_counter()
             for s_ in pool:
                 sv.solve(s_, sp)
             seq_ms.append((time.perf_counter() - t1) * 1e3 / len(pool))
+        serve_keys = ("delta", "buckets", "bucket_window", "l2_batch", "min_read", "l1_pop",
+                      "light_split")
+        base = {k: v for k, v in single.items() if k in serve_keys}
+        if args.tune:
+            t = tune_serving(server, gid, pool, base=base, log=log)
+            tuned.append(t["params"])
+            log(f"serve graph {gid} tuned: {t['params']} {t['qps']:.1f} q/s")
+        else:
+            server.set_graph_params(gid, **base)
+            tuned.append(base)
         graphs.append({"name": name, "vertices": g.n, "edges": g.m, "params": tuned[-1],
                        "slot_mib": round(server.slot_bytes(gid) / 2**20, 2),
                        "plain_solve_ms": round(seq_ms[-1], 3)})

@@ -102,6 +102,18 @@ typedef struct mlmq_params {
                                  traffic cost more than the round trip they save) */
     int32_t  idle_backoff;    /* longest idle sleep between polls of an empty queue, in units of
                                  256 cycles, 1..64; 0 = 16 */
+    int32_t  defer_heavy;     /* 1: a popped record with more than 7 out-edges whose distance lies
+                                 beyond the head bucket goes back to L2 instead of being expanded
+                                 (power-law graphs: hubs expand in bucket order); 0 (default) */
+    int32_t  strict_window;   /* 1: the bucket window advances only when every record taken from
+                                 L2 has been credited back (a delta-stepping phase barrier);
+                                 0 (default): as soon as the head bucket is drained */
+    int32_t  light_split;     /* 1: light/heavy edge split (delta-stepping's): a record relaxes its
+                                 vertex's edges of weight < delta at once and queues one
+                                 heavy-phase record, one bucket later, that relaxes the edges of
+                                 weight >= delta; needs a graph loaded through mlmq_load_csr /
+                                 mlmq_load_gr (adjacency sorted by weight); 0 (default) */
+    int32_t  reserved[1];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */

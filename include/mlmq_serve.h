@@ -75,7 +75,8 @@ typedef struct mlmq_serve_graph_params {
                                  Doubled automatically when a request overflows (the request
                                  is then run again). */
     int32_t  hub_split;       /* 1: split hub edge lists into chunk records (mlmq_params.hub_split) */
-    int32_t  reserved[7];
+    int32_t  light_split;     /* 1: light/heavy edge split (mlmq_params.light_split); excludes hub_split */
+    int32_t  reserved[6];
 } mlmq_serve_graph_params;
 
 typedef struct mlmq_serve_request {

@@ -62,6 +62,8 @@ class Params(ctypes.Structure):
         ("use_graph", ctypes.c_int32), ("hub_split", ctypes.c_int32),
         ("stale_check", ctypes.c_int32), ("prefetch", ctypes.c_int32),
         ("l1_blocks", ctypes.c_int32), ("idle_backoff", ctypes.c_int32),
+        ("defer_heavy", ctypes.c_int32), ("strict_window", ctypes.c_int32),
+        ("light_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -134,7 +136,8 @@ class ServeGraphParams(ctypes.Structure):
         ("min_read", ctypes.c_int32), ("l1_pop", ctypes.c_int32),
         ("l1_spill", ctypes.c_int32), ("workgroups", ctypes.c_int32),
         ("waves", ctypes.c_int32), ("bucket_capacity", ctypes.c_int64),
-        ("hub_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("hub_split", ctypes.c_int32), ("light_split", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

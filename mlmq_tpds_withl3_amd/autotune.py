@@ -76,8 +76,9 @@ def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int
     best_s = score(cur)
     trail = [(dict(cur), best_s)]
     space = [("delta", _delta_grid(cur["delta"])), ("workgroups", [1, 2, 4, 8]),
-             ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64]), ("buckets", [16, 4, 2]),
-             ("min_read", [8, 16, 32])]
+             ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
+    if not cur.get("light_split"):
+        space.append(("buckets", [16, 4, 2]))
     for key, values in space:
         for v in values:
             if cur.get(key) == v:

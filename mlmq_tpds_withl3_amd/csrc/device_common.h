@@ -144,4 +144,13 @@ __device__ __forceinline__ uint64_t make_rec(uint32_t v, uint32_t d) {
 __device__ __forceinline__ uint32_t rec_vertex(uint64_t r) { return (uint32_t)r; }
 __device__ __forceinline__ uint32_t rec_dist(uint64_t r) { return (uint32_t)(r >> 32); }
 
+// Light/heavy edge split: a heavy-phase record (relax only the edges of weight
+// >= delta of its vertex) carries this flag in the top bit of its distance word
+// (distances are below 2^31) and is queued one bucket later than its distance.
+constexpr uint32_t kHeavyPhase = 0x80000000u;
+__device__ __forceinline__ uint32_t bucket_dist(uint64_t r, uint32_t delta) {
+    const uint32_t d = rec_dist(r);
+    return (d & kHeavyPhase) ? (d & ~kHeavyPhase) + delta : d;
+}
+
 }  // namespace mlmq

@@ -42,6 +42,7 @@ struct L2Queue {
     uint32_t delta;      // bucket width (> 0)
     uint32_t batch;      // claim size, 1..64
     uint32_t spin_limit; // bounded wait on a full slot
+    uint32_t strict;     // 1: the window advances only once every taken record is credited
 };
 
 struct WaveStats {
@@ -68,7 +69,7 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
     const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
     uint32_t b = 0;
     if (valid) {
-        int32_t rel = (int32_t)(rec_dist(rec) / q.delta) - (int32_t)base;
+        int32_t rel = (int32_t)(bucket_dist(rec, q.delta) / q.delta) - (int32_t)base;
         rel = rel < 0 ? 0 : rel;
         rel = rel >= (int32_t)q.nbuckets ? (int32_t)q.nbuckets - 1 : rel;
         b = (base + (uint32_t)rel) % q.nbuckets;
@@ -116,6 +117,7 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
         k = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
     }
     const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
+    const uint32_t done = q.strict ? uniform(ld_relaxed(q.ctl + kDoneOff)) : 0u;
     const bool unclaimed = mine && (int32_t)(t - h) > 0;
     const uint64_t avail_mask = ballot(unclaimed);
     const uint64_t claim_mask = ballot(mine && cl_mask != 0);
@@ -154,12 +156,17 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
         }
     }
     // Window advance: head bucket drained (every reserved record taken) while
-    // other buckets still hold records.
+    // other buckets still hold records.  Strict mode also waits until every
+    // taken record has been credited back (its expansion, and the L1 work it
+    // spawned, finished), so no record of the head bucket can still lower a
+    // distance in the next one: a delta-stepping phase barrier.
     const uint32_t b0 = base % q.nbuckets;
     const uint32_t live = mine ? t - k : 0;
     if (read_lane(live, b0) == 0 && !((claim_mask >> b0) & 1)) {
         const uint32_t total = uniform(wave_sum(live));
-        if (total > 0 && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
+        bool advance = total > 0;
+        if (q.strict) advance = advance && done == uniform(wave_sum(mine ? k : 0u));
+        if (advance && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
     }
     st.v[kStatL2Pop] += got;
     wave_sync();

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device_common.h"
@@ -33,6 +34,8 @@ struct mlmq_ctx {
     uint4 *blocks = nullptr;   // hop blocks, always owned by the context
     uint2 *hubs = nullptr;     // hub table (vertex, chunk size), owned
     uint32_t nhubs = 0;
+    bool sorted = false;       // every adjacency sorted by weight (owned uploads)
+    uint32_t light_delta = 0;  // delta the blocks' light-edge counts were computed for
     bool owns_graph = false;
     mlmq_graph_info info{};
     // per-query state
@@ -121,6 +124,8 @@ void free_graph(mlmq_ctx *c) {
     if (c->hubs) (void)hipFree(c->hubs);
     c->hubs = nullptr;
     c->nhubs = 0;
+    c->sorted = false;
+    c->light_delta = 0;
     c->row = c->col = c->wgt = nullptr;
     c->owns_graph = false;
     c->n = 0;
@@ -231,6 +236,41 @@ int upload(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t 
     std::vector<uint32_t> row32((size_t)n + 1);
     for (int64_t i = 0; i <= n; ++i) row32[i] = (uint32_t)row[i];
     const size_t me = (size_t)(m > 0 ? m : 1);
+    // The adjacency of every vertex with more than kInlineEdges edges (the
+    // ones relaxed through the CSR) is sorted by (weight, destination): the
+    // order in which a vertex's edges are relaxed does not change any
+    // distance, lighter edges first lowers fewer distances twice, and the
+    // edges lighter than any delta form a prefix (light/heavy split).
+    // Inline vertices keep their order (their split is decided per edge).
+    std::vector<int32_t> scol((size_t)m), sw((size_t)m);
+    {
+        const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t]() {
+                std::vector<std::pair<int32_t, int32_t>> tmp;
+                for (int64_t u = t; u < n; u += nt) {
+                    const int64_t b = row[u], e = row[u + 1];
+                    if (e - b <= (int64_t)mlmq::kInlineEdges) {
+                        for (int64_t i = b; i < e; ++i) {
+                            sw[(size_t)i] = w[i];
+                            scol[(size_t)i] = col[i];
+                        }
+                        continue;
+                    }
+                    tmp.resize((size_t)(e - b));
+                    for (int64_t i = b; i < e; ++i) tmp[(size_t)(i - b)] = {w[i], col[i]};
+                    std::sort(tmp.begin(), tmp.end());
+                    for (int64_t i = b; i < e; ++i) {
+                        sw[(size_t)i] = tmp[(size_t)(i - b)].first;
+                        scol[(size_t)i] = tmp[(size_t)(i - b)].second;
+                    }
+                }
+            });
+        for (auto &x : th) x.join();
+    }
+    col = scol.data();
+    w = sw.data();
     HIP_TRY(c, hipMalloc(&c->row, sizeof(uint32_t) * ((size_t)n + 1)));
     c->owns_graph = true;
     HIP_TRY(c, hipMalloc(&c->col, sizeof(uint32_t) * me));
@@ -244,7 +284,19 @@ int upload(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const int32_t 
     c->n = n;
     c->m = m;
     compute_info(&c->info, n, m, row, nullptr, w);
+    c->sorted = true;
     return build_blocks(c);
+}
+
+// Light-edge counts in the hop blocks for `delta` (light/heavy split).
+int ensure_light_counts(mlmq_ctx *c, uint32_t delta) {
+    if (!c->sorted) return fail(c, MLMQ_E_UNSUPPORTED, "light_split needs a graph loaded by mlmq_load_csr/gr");
+    if (c->light_delta == delta) return MLMQ_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, mlmq::launch_light_count(c->row, c->wgt, (uint32_t)c->n, delta, c->blocks, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->light_delta = delta;
+    return MLMQ_OK;
 }
 
 uint32_t ceil_log2(uint64_t x) {
@@ -287,6 +339,12 @@ int ensure_queue(mlmq_ctx *c, uint32_t nbuckets, uint32_t cap_log2) {
 }
 
 }  // namespace
+
+int mlmq::ctx_prepare_light(mlmq_ctx *c, uint32_t delta) {
+    if (!c) return MLMQ_E_INVALID;
+    if (!c->row) return MLMQ_E_NOGRAPH;
+    return ensure_light_counts(c, delta);
+}
 
 int mlmq::ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v) {
     if (!c || !v) return MLMQ_E_INVALID;
@@ -333,6 +391,9 @@ void mlmq_params_default(mlmq_params *p) {
     p->prefetch = 1;
     p->l1_blocks = 0;
     p->idle_backoff = 16;
+    p->defer_heavy = 0;
+    p->strict_window = 0;
+    p->light_split = 0;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -512,6 +573,12 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
     if (p.l1_blocks != 0 && p.l1_blocks != 1) return fail(c, MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
+    if (p.light_split && (p.hub_split || p.defer_heavy))
+        return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
+    if (p.light_split) {
+        const int lrc = ensure_light_counts(c, p.delta);
+        if (lrc) return lrc;
+    }
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
         return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -563,6 +630,9 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         L.split = p.hub_split != 0;
         L.stale_check = (uint32_t)p.stale_check;
         L.prefetch = p.prefetch ? 1u : 0u;
+        L.defer_heavy = p.defer_heavy ? 1u : 0u;
+        L.strict_window = p.strict_window ? 1u : 0u;
+        L.light_split = p.light_split ? 1u : 0u;
         L.hot = hot;
         L.dist = c->dist;
         L.slots = c->slots;
@@ -675,6 +745,12 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         return fail(c, MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
     if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
+    if (p.light_split && (p.hub_split || p.defer_heavy))
+        return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
+    if (p.light_split) {
+        const int lrc = ensure_light_counts(c, p.delta);
+        if (lrc) return lrc;
+    }
     if (p.work_blocks < 0 || p.bucket_capacity < 0) return fail(c, MLMQ_E_INVALID, "negative size parameter");
     for (int32_t i = 0; i < nq; ++i)
         if (sources[i] < 0 || sources[i] >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
@@ -750,6 +826,9 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.split = p.hub_split != 0;
         L.stale_check = (uint32_t)p.stale_check;
         L.prefetch = p.prefetch ? 1u : 0u;
+        L.defer_heavy = p.defer_heavy ? 1u : 0u;
+        L.strict_window = p.strict_window ? 1u : 0u;
+        L.light_split = p.light_split ? 1u : 0u;
         L.hot = false;
         L.dist = nullptr;
         L.slots = nullptr;
@@ -792,7 +871,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             (uint64_t)(uintptr_t)c->b_ctl, (uint64_t)(uintptr_t)c->b_sources,
             (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks,
             (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check,
-            (uint64_t)L.prefetch, (uint64_t)L.max_sleep};
+            (uint64_t)L.prefetch, (uint64_t)L.max_sleep, (uint64_t)L.defer_heavy,
+            (uint64_t)L.strict_window, (uint64_t)L.light_split};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

@@ -122,6 +122,8 @@ int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n,
     if (p.waves != 4 && p.waves != 8 && p.waves != 16) return fail(s, MLMQ_E_INVALID, "waves must be 4, 8 or 16");
     if (p.bucket_capacity < 0) return fail(s, MLMQ_E_INVALID, "negative bucket_capacity");
     if (p.hub_split != 0 && p.hub_split != 1) return fail(s, MLMQ_E_INVALID, "hub_split must be 0 or 1");
+    if (p.light_split != 0 && p.light_split != 1) return fail(s, MLMQ_E_INVALID, "light_split must be 0 or 1");
+    if (p.light_split && p.hub_split) return fail(s, MLMQ_E_INVALID, "light_split excludes hub_split");
     uint64_t cap = (uint64_t)p.bucket_capacity;
     if (cap == 0) cap = std::max<uint64_t>((uint64_t)n / 32, 8192);
     const uint32_t c = ceil_log2(cap);
@@ -193,6 +195,10 @@ int add_graph_ctx(mlmq_server *s, mlmq_ctx *ctx, const mlmq_serve_graph_params *
     g.ctx = ctx;
     int rc = mlmq::ctx_graph_view(ctx, &g.v);
     if (rc == MLMQ_OK) rc = resolve_params(s, p, g.v.n, g.v.m, &g.p, &g.cap_log2);
+    if (rc == MLMQ_OK && g.p.light_split) {
+        rc = mlmq::ctx_prepare_light(ctx, g.p.delta);
+        if (rc) s->err = mlmq_last_error(ctx);
+    }
     if (rc != MLMQ_OK) {
         mlmq_destroy(ctx);
         return rc;
@@ -340,6 +346,10 @@ int mlmq_server_set_graph_params(mlmq_server *s, int32_t graph_id, const mlmq_se
     uint32_t c = 0;
     int rc = resolve_params(s, p, g.v.n, g.v.m, &np, &c);
     if (rc) return rc;
+    if (np.light_split) {
+        rc = mlmq::ctx_prepare_light(g.ctx, np.delta);
+        if (rc) return fail(s, rc, mlmq_last_error(g.ctx));
+    }
     drop_free_slots(s, g);
     g.p = np;
     g.cap_log2 = c;
@@ -484,6 +494,9 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.stale_check = 2;
         L.prefetch = 1;
         L.hot = false;
+        L.defer_heavy = 0;
+        L.strict_window = 0;
+        L.light_split = g.p.light_split ? 1u : 0u;
         L.n = (uint32_t)g.v.n;
         L.nbuckets = (uint32_t)g.p.buckets;
         L.window = (uint32_t)g.p.bucket_window;

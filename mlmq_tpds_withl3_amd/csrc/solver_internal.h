@@ -24,6 +24,8 @@ struct DeviceGraphView {
     int wall_khz;
 };
 int ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v);
+// Light-edge counts of the context's hop blocks for `delta` (light/heavy split).
+int ctx_prepare_light(mlmq_ctx *c, uint32_t delta);
 
 constexpr int kWavesPerBlock = 4;   // 256-thread persistent workgroups
 
@@ -62,6 +64,9 @@ struct SolverLaunch {
     uint32_t stale_check;   // 2 coherent, 1 cached, 0 none
     uint32_t prefetch;      // 1: prefetch relaxed children's hop blocks
     bool hot;               // with slf: block-carrying L1 (single-query kernel only)
+    uint32_t defer_heavy;   // 1: defer heavy records beyond the head bucket
+    uint32_t strict_window; // 1: window advance waits for all taken records to be credited
+    uint32_t light_split;   // 1: light/heavy edge split (sorted adjacency, light counts set)
     bool profile;   // per-phase clock build of the kernel
 };
 
@@ -112,6 +117,8 @@ hipError_t launch_reset(uint32_t *dist, uint32_t n, uint32_t src, uint32_t *ctl,
 hipError_t launch_fill_u64(uint64_t *p, uint64_t count, uint64_t value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);
 hipError_t launch_patch_hubs(const uint2 *hubs, uint32_t nh, uint4 *blocks, hipStream_t s);
+hipError_t launch_light_count(const uint32_t *row, const uint32_t *wgt, uint32_t n, uint32_t delta,
+                              uint4 *blocks, hipStream_t s);
 hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
                                uint32_t n, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);

@@ -40,6 +40,8 @@ struct SolveArgs {
     uint32_t split;         // 1: split hub edge lists into chunk records
     uint32_t stale_check;   // 2: coherent dist load, 1: cached load, 0: none
     uint32_t prefetch;      // 1: touch each relaxed child's hop block during the atomics
+    uint32_t defer_heavy;   // 1: heavy records beyond the head bucket go back to L2
+    uint32_t light_split;   // 1: relax light edges now, heavy edges (w >= delta) one bucket later
     uint64_t timeout_ticks;
 };
 
@@ -198,7 +200,10 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     const uint32_t lane = lane_id();
     const bool valid = lane < nin;
     uint32_t v = rec_vertex(rec);
-    const uint32_t d = rec_dist(rec);
+    // Light/heavy split: a heavy-phase record relaxes only its vertex's edges
+    // of weight >= delta (the reference has no such split; see SolveArgs).
+    const bool hphase = a.light_split && (rec_dist(rec) & kHeavyPhase) != 0;
+    const uint32_t d = rec_dist(rec) & ~kHeavyPhase;
     // A chunk record names (hub id, chunk index); one extra round trip to the
     // hub table, taken only by batches that hold chunks.
     const bool chunk = valid && (v & kChunkFlag) != 0;
@@ -234,8 +239,21 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     }
     // idle-wave count for work donation, in the same round trip
     hungry = a.donate_min ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
-    const bool keep = valid && (hot || d <= cur);
+    // bucket epoch, for deferring heavy records (same round trip)
+    const uint32_t epoch = a.defer_heavy ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
+    bool keep = valid && (hot || d <= cur);
     st.v[kStatStale] += popc(ballot(valid && !keep));
+    // Deferral: a heavy record whose distance lies beyond the head bucket goes
+    // back to L2 (into its own bucket) instead of being expanded now, so hub
+    // edge lists are expanded in bucket order and rarely twice.
+    if (a.defer_heavy) {
+        const bool defer = keep && !chunk && b.deg() > kInlineEdges && d / a.q.delta > epoch;
+        if (ballot(defer)) {
+            st.v[kStatSpare1] += popc(ballot(defer));
+            if (!l2_push(a.q, make_rec(v, d), defer, st)) return false;
+            keep = keep && !defer;
+        }
+    }
     uint32_t deg = keep ? b.deg() : 0u;
     uint32_t hstart = b.start();
     uint32_t nextra = 0;   // chunks of a hub popped here that go to L2
@@ -249,18 +267,40 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         deg = min(deg, csize);
     }
     // chunk lanes always take the CSR path (their edges are not inline)
-    const bool heavy = chunk ? deg > 0 : deg > kInlineEdges;
-    const uint32_t ideg = heavy ? 0u : deg;
+    const bool csr_path = chunk || b.deg() > kInlineEdges;
+    bool need_heavy = false;   // a light-phase record whose vertex has heavy edges
+    if (a.light_split && csr_path && !chunk) {
+        // adjacency sorted by weight: the first b.q[1].x edges are light
+        const uint32_t nl = min(b.q[1].x, deg);
+        if (hphase) {
+            hstart += nl;
+            deg -= nl;
+        } else {
+            need_heavy = deg > nl;
+            deg = nl;
+        }
+    }
+    const bool heavy = csr_path && deg > 0;
+    const uint32_t ideg = csr_path ? 0u : deg;
     clk.lap(st, kClkRow);
     if (ballot(nextra > 0)) {
         if (!push_chunks(a, hub_word - 1, d, nextra, map, st)) return false;
     }
 
     uint32_t c[kInlineEdges], nd[kInlineEdges], pv[kInlineEdges];
+    uint32_t on = 0;   // bit k: inline edge k is relaxed by this record
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         c[k] = word_of(b, 2 + 2 * k);
-        nd[k] = d + word_of(b, 3 + 2 * k);
+        const uint32_t wk = word_of(b, 3 + 2 * k);
+        nd[k] = d + wk;
+        bool e = (uint32_t)k < ideg;
+        if (a.light_split) {
+            const bool hv = wk >= a.q.delta;
+            need_heavy = need_heavy || (e && hv && !hphase);
+            e = e && hv == hphase;
+        }
+        if (e) on |= 1u << k;
     }
     // Prefetch, in the same round trip as the atomics: touch every relaxed
     // child's hop block, so the batch that later pops the child finds the
@@ -275,18 +315,18 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     if (kHot) {
 #pragma unroll
         for (int k = 0; k < (kHot ? (int)kInlineEdges : 0); ++k)
-            cb[k] = load_block(a.blocks, (uint32_t)k < ideg ? c[k] : 0u);
+            cb[k] = load_block(a.blocks, ((on >> k) & 1) ? c[k] : 0u);
     } else if (a.prefetch) {
 #pragma unroll
         for (int k = 0; k < (int)kInlineEdges; ++k) {
-            const uint32_t t = (uint32_t)k < ideg ? c[k] : 0u;
+            const uint32_t t = ((on >> k) & 1) ? c[k] : 0u;
             pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
         }
     }
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         pv[k] = 0;
-        if ((uint32_t)k < ideg) pv[k] = fetch_min(a.dist + c[k], nd[k]);
+        if ((on >> k) & 1) pv[k] = fetch_min(a.dist + c[k], nd[k]);
     }
     // Hot lanes' stale check, in the same round trip.  A record whose vertex
     // has since dropped below d has a fresher record queued; that one relaxes
@@ -304,7 +344,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t bits = 0;   // bit k: edge k lowered its destination's distance
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
-        if ((uint32_t)k < ideg && nd[k] < pv[k]) bits |= 1u << k;
+        if (((on >> k) & 1) && nd[k] < pv[k]) bits |= 1u << k;
         out[k] = make_rec(c[k], nd[k]);
     }
     if (kHot) {
@@ -313,7 +353,8 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         if (late_stale) bits = 0;
     }
     // relaxations and improvements of the batch in one wave reduction
-    const uint32_t sums = wave_sum(ideg | ((uint32_t)__builtin_popcount(bits) << 16));
+    const uint32_t sums = wave_sum((uint32_t)__builtin_popcount(on) |
+                                   ((uint32_t)__builtin_popcount(bits) << 16));
     st.v[kStatRelax] += sums & 0xFFFFu;
     st.v[kStatImprove] += sums >> 16;
     clk.lap(st, kClkAtomic);
@@ -334,6 +375,10 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         ok = relax_heavy<kSLF, kHot ? 1 : 0>(a, heavy, d, hstart, deg, ring, blk, l1, map, st);
         clk.lap(st, kClkEdges);
     }
+    // The heavy edges wait for a heavy-phase record one bucket later: by then
+    // the light phase of this bucket has settled the vertex's distance, so
+    // the heavy edges are relaxed once instead of once per improvement.
+    if (ok && ballot(need_heavy)) ok = l2_push(a.q, make_rec(v, d | kHeavyPhase), need_heavy, st);
     return ok;
 }
 
@@ -695,6 +740,25 @@ __global__ void build_blocks_kernel(const uint32_t *row, const uint32_t *col, co
     }
 }
 
+// Light-edge count for the light/heavy split: for every vertex with more
+// than kInlineEdges out-edges (adjacency sorted by weight), word 4 of its
+// block = number of edges of weight < delta (binary search).
+__global__ void light_count_kernel(const uint32_t *row, const uint32_t *wgt, uint32_t n,
+                                   uint32_t delta, uint4 *blocks) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint32_t s = row[v], e = row[v + 1];
+        if (e - s <= kInlineEdges) continue;
+        uint32_t lo = s, hi = e;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (wgt[mid] < delta) lo = mid + 1;
+            else hi = mid;
+        }
+        blocks[4ull * v + 1].x = lo - s;
+    }
+}
+
 // Mark hubs in their hop blocks: word 2 = hub id + 1, word 3 = chunk size.
 __global__ void patch_hubs_kernel(const uint2 *hubs, uint32_t nh, uint4 *blocks) {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -775,6 +839,13 @@ hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const u
     return hipGetLastError();
 }
 
+hipError_t launch_light_count(const uint32_t *row, const uint32_t *wgt, uint32_t n, uint32_t delta,
+                              uint4 *blocks, hipStream_t s) {
+    hipLaunchKernelGGL(light_count_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, row, wgt, n, delta,
+                       blocks);
+    return hipGetLastError();
+}
+
 hipError_t launch_patch_hubs(const uint2 *hubs, uint32_t nh, uint4 *blocks, hipStream_t s) {
     if (nh == 0) return hipSuccess;
     hipLaunchKernelGGL(patch_hubs_kernel, dim3(grid_for(nh, 256)), dim3(256), 0, s, hubs, nh, blocks);
@@ -826,6 +897,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.q.delta = L.delta;
     a.q.batch = L.batch;
     a.q.spin_limit = L.spin_limit;
+    a.q.strict = L.strict_window;
     a.n = L.n;
     a.min_read = L.min_read;
     a.max_sleep = L.max_sleep;
@@ -836,6 +908,8 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.split = L.hubs && L.split ? 1u : 0u;
     a.stale_check = L.stale_check;
     a.prefetch = L.prefetch;
+    a.defer_heavy = L.defer_heavy;
+    a.light_split = L.light_split;
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }

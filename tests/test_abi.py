@@ -47,7 +47,8 @@ def test_loader_binds_all():
         (19, 200000, 16, 4, 8, 8)
     assert (p.l1_pop, p.l1_spill, p.l1_donate, p.profile) == (32, 64, 0, 0)
     assert (p.batch_waves, p.use_graph, p.hub_split, p.stale_check, p.prefetch, p.l1_blocks,
-            p.idle_backoff) == (8, 1, 0, 2, 1, 0, 16)
+            p.idle_backoff, p.defer_heavy, p.strict_window, p.light_split) == \
+        (8, 1, 0, 2, 1, 0, 16, 0, 0, 0)
 
 
 def test_library_is_gfx950_and_independent_of_oracle():
@@ -82,12 +83,14 @@ def test_ctypes_layouts_match_c(tmp_path):
     C compiler gives it."""
     structs = {
         "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph", "hub_split",
-                                       "stale_check", "prefetch", "l1_blocks", "idle_backoff"]),
+                                       "stale_check", "prefetch", "l1_blocks", "idle_backoff", "defer_heavy",
+                                       "strict_window", "light_split", "reserved"]),
         "mlmq_timing": (_lib.Timing, ["donated", "hub_chunks"]),
         "mlmq_batch_timing": (_lib.BatchTiming, None),
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
         "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "reserved"]),
-        "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "reserved"]),
+        "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "light_split",
+                                                              "reserved"]),
         "mlmq_serve_request": (_lib.ServeRequest, ["src", "dist_out"]),
         "mlmq_serve_result": (_lib.ServeResult, ["status", "retries"]),
         "mlmq_serve_stats": (_lib.ServeStats, ["requests", "peak_inflight", "reserved"]),

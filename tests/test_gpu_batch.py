@@ -84,6 +84,22 @@ def test_stale_modes_batch(solver, stale):
     check_batch(solver, row, col, w, [0, 5599, 123, 4000], work_blocks=2, stale_check=stale)
 
 
+def test_light_split_batch(solver):
+    say("light split batch")
+    row, col, w = hub_csr(20000, [40000, 5000, 2100], 40000, seed=10)
+    solver.load_csr(row, col, w)
+    check_batch(solver, row, col, w, [0, 1, 19999, 77, 5], work_blocks=2, light_split=1, delta=100,
+                buckets=32)
+
+
+def test_strict_window_batch(solver):
+    say("strict window batch")
+    row, col, w = hub_csr(20000, [40000, 5000, 2100], 40000, seed=9)
+    solver.load_csr(row, col, w)
+    check_batch(solver, row, col, w, [0, 1, 19999, 77], work_blocks=3, strict_window=1, delta=100,
+                buckets=32, bucket_window=1)
+
+
 def test_graph_replays_interleaved(solver):
     say("graph replays with eager work in between")
     row, col, w = grid_csr(80, 90, 21)

@@ -65,6 +65,7 @@ def test_random_graphs(solver, seed, queue):
     for src in (0, n - 1):
         for blocks in ((1, 0) if queue != "L1N_L2DQ" else (1,)):
             solve_check(solver, row, col, w, src, queue=queue, delta=300, l1_blocks=blocks)
+        solve_check(solver, row, col, w, src, load=False, queue=queue, delta=300, light_split=1)
 
 
 @pytest.mark.parametrize("cfg", [
@@ -85,6 +86,12 @@ def test_random_graphs(solver, seed, queue):
     dict(l1_blocks=1, delta=2000),                               # block-carrying L1
     dict(l1_blocks=1, queue="L1V_L2DQ"),
     dict(l1_blocks=1, stale_check=0, l1_pop=8, min_read=8),      # hot pops, no stale filter
+    dict(defer_heavy=1, delta=300),
+    dict(strict_window=1, delta=300),                           # phase barrier per bucket
+    dict(light_split=1, delta=300),                             # light/heavy edge split
+    dict(light_split=1, delta=1, buckets=64, bucket_window=2),  # only zero weights are light
+    dict(light_split=1, delta=100, queue="L1N_L2DQ", strict_window=1),
+    dict(strict_window=1, delta=100, bucket_window=1, queue="L1N_L2DQ"),
 ])
 def test_grid_param_corners(solver, cfg):
     say(f"grid 120x150 {cfg}")
@@ -104,6 +111,12 @@ HUBS = [70000, 9000, 4097, 2049, 2048, 1500]   # degrees 70002 .. 1504: five hub
     dict(delta=100, l2_batch=64, min_read=64),
     dict(l1_spill=4, l1_pop=8),
     dict(l1_blocks=1),
+    dict(defer_heavy=1, delta=50),                  # hubs expand only in the head bucket
+    dict(defer_heavy=1, delta=50, buckets=4, bucket_window=4, queue="L1N_L2DQ"),
+    dict(strict_window=1, delta=50, buckets=64, bucket_window=1),
+    dict(light_split=1, delta=100, hub_split=0),
+    dict(light_split=1, delta=20, buckets=64, bucket_window=4, hub_split=0),
+    dict(light_split=1, delta=500, queue="L1V_L2DQ", hub_split=0),
 ])
 def test_hub_split(solver, cfg):
     say(f"hubs {cfg}")
@@ -140,7 +153,8 @@ def test_road264k(solver):
     g, src, spec = make_workload("road264k")
     row, col, w = g.arrays()
     for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ"),
-                dict(stale_check=1), dict(stale_check=0), dict(l1_blocks=1)):
+                dict(stale_check=1), dict(stale_check=0), dict(l1_blocks=1),
+                dict(light_split=1, delta=20000)):
         say(f"road264k {cfg}")
         t, work, reached = solve_check(solver, row, col, w, src, **cfg)
         assert t["relaxations"] >= work

@@ -171,10 +171,10 @@ def test_hub_graph_served(graphs):
     say("hub graph served")
     g = G(*hub_csr(20000, [40000, 5000, 2100], 40000, seed=8))
     s, ids = new_server([g], streams=2, max_batch=4)
-    for split in (1, 0):
-        s.set_graph_params(0, delta=300, workgroups=2, hub_split=split)
+    for prm in (dict(hub_split=1), dict(hub_split=0), dict(light_split=1, delta=100, buckets=32)):
+        s.set_graph_params(0, **dict(dict(delta=300, workgroups=2), **prm))
         run_checked(s, [g], np.linspace(0, 0.002, 6), np.zeros(6, dtype=np.int32),
-                    np.array([0, 1, 2, 19999, 5, 0], dtype=np.int32), f"hubs split={split}")
+                    np.array([0, 1, 2, 19999, 5, 0], dtype=np.int32), f"hubs {prm}")
     s.close()
 
 
