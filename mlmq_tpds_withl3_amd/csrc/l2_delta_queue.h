@@ -104,6 +104,7 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
 // Pop up to one claim's worth (at most `room`) of ready records into
 // `stage[0..got)`.  `cl_start`/`cl_mask` carry the wave's claims: lane b
 // holds bucket b.
+template <bool kStrict = false>
 __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
                            uint64_t *stage, uint32_t room, WaveStats &st) {
     const uint32_t lane = lane_id();
@@ -117,7 +118,7 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
         k = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
     }
     const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
-    const uint32_t done = q.strict ? uniform(ld_relaxed(q.ctl + kDoneOff)) : 0u;
+    const uint32_t done = kStrict && q.strict ? uniform(ld_relaxed(q.ctl + kDoneOff)) : 0u;
     const bool unclaimed = mine && (int32_t)(t - h) > 0;
     const uint64_t avail_mask = ballot(unclaimed);
     const uint64_t claim_mask = ballot(mine && cl_mask != 0);
@@ -165,7 +166,7 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
     if (read_lane(live, b0) == 0 && !((claim_mask >> b0) & 1)) {
         const uint32_t total = uniform(wave_sum(live));
         bool advance = total > 0;
-        if (q.strict) advance = advance && done == uniform(wave_sum(mine ? k : 0u));
+        if (kStrict && q.strict) advance = advance && done == uniform(wave_sum(mine ? k : 0u));
         if (advance && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
     }
     st.v[kStatL2Pop] += got;

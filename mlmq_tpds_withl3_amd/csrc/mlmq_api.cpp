@@ -575,6 +575,8 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.light_split && (p.hub_split || p.defer_heavy))
         return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
+    if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
+        return fail(c, MLMQ_E_INVALID, "l1_blocks excludes light_split, defer_heavy and strict_window");
     if (p.light_split) {
         const int lrc = ensure_light_counts(c, p.delta);
         if (lrc) return lrc;
@@ -747,6 +749,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.light_split && (p.hub_split || p.defer_heavy))
         return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
+    if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
+        return fail(c, MLMQ_E_INVALID, "l1_blocks excludes light_split, defer_heavy and strict_window");
     if (p.light_split) {
         const int lrc = ensure_light_counts(c, p.delta);
         if (lrc) return lrc;

@@ -192,7 +192,7 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 //      back (up to 7 per lane) before any result is used.
 // A (dst, nd) record is pushed only if its atomicMin strictly lowered dst's
 // distance (:211-243).  The batch's pushes go to L1 in one pass.
-template <bool kSLF, bool kProf, bool kHot>
+template <bool kSLF, bool kProf, bool kHot, bool kExt>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
                               const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
                               uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &hungry,
@@ -202,7 +202,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t v = rec_vertex(rec);
     // Light/heavy split: a heavy-phase record relaxes only its vertex's edges
     // of weight >= delta (the reference has no such split; see SolveArgs).
-    const bool hphase = a.light_split && (rec_dist(rec) & kHeavyPhase) != 0;
+    const bool hphase = (kExt && a.light_split) && (rec_dist(rec) & kHeavyPhase) != 0;
     const uint32_t d = rec_dist(rec) & ~kHeavyPhase;
     // A chunk record names (hub id, chunk index); one extra round trip to the
     // hub table, taken only by batches that hold chunks.
@@ -240,13 +240,13 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // idle-wave count for work donation, in the same round trip
     hungry = a.donate_min ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
     // bucket epoch, for deferring heavy records (same round trip)
-    const uint32_t epoch = a.defer_heavy ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
+    const uint32_t epoch = (kExt && a.defer_heavy) ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
     bool keep = valid && (hot || d <= cur);
     st.v[kStatStale] += popc(ballot(valid && !keep));
     // Deferral: a heavy record whose distance lies beyond the head bucket goes
     // back to L2 (into its own bucket) instead of being expanded now, so hub
     // edge lists are expanded in bucket order and rarely twice.
-    if (a.defer_heavy) {
+    if (kExt && a.defer_heavy) {
         const bool defer = keep && !chunk && b.deg() > kInlineEdges && d / a.q.delta > epoch;
         if (ballot(defer)) {
             st.v[kStatSpare1] += popc(ballot(defer));
@@ -269,7 +269,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // chunk lanes always take the CSR path (their edges are not inline)
     const bool csr_path = chunk || b.deg() > kInlineEdges;
     bool need_heavy = false;   // a light-phase record whose vertex has heavy edges
-    if (a.light_split && csr_path && !chunk) {
+    if ((kExt && a.light_split) && csr_path && !chunk) {
         // adjacency sorted by weight: the first b.q[1].x edges are light
         const uint32_t nl = min(b.q[1].x, deg);
         if (hphase) {
@@ -295,7 +295,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         const uint32_t wk = word_of(b, 3 + 2 * k);
         nd[k] = d + wk;
         bool e = (uint32_t)k < ideg;
-        if (a.light_split) {
+        if ((kExt && a.light_split)) {
             const bool hv = wk >= a.q.delta;
             need_heavy = need_heavy || (e && hv && !hphase);
             e = e && hv == hphase;
@@ -386,7 +386,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 // loop, SS/SSSP/sssp_run.cu:507-584).  Returns when the query's exit flag is
 // seen, after an error, or at the deadline.  `leader` waves also run the
 // termination check while idle.
-template <bool kSLF, bool kProf, bool kHot = false>
+template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false>
 __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
                            uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
                            uint64_t deadline, HopBlock *blk = nullptr) {
@@ -419,7 +419,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         // cold records behind a hot run; it tops up only once it is empty, so
         // a hot batch stays free of global loads.
         if (nin < a.min_read && (!kHot || l1.size == 0)) {
-            const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, kWave - nin, st);
+            const uint32_t got = l2_pop<kExt>(q, cl_start, cl_mask, stage, kWave - nin, st);
             if (got) {
                 credit += got;
                 if (lane >= nin && lane < nin + got) rec = stage[lane - nin];
@@ -455,7 +455,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf, kHot>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
+        if (!process_batch<kSLF, kProf, kHot, kExt>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
                                               st, clk))
             break;
         if (kSLF && hungry && l1.size > a.donate_min) {
@@ -499,7 +499,7 @@ __device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
 // kHot: block-carrying L1 (each wave's ring paired with 512 hop-block slots in
 // LDS, 32 KiB per wave; about 150 KiB per 4-wave workgroup, so one
 // workgroup per CU).
-template <int kWaves, bool kSLF, bool kProf, bool kHot>
+template <int kWaves, bool kSLF, bool kProf, bool kHot, bool kExt = false>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
     __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
         for (int i = 0; i < 16; ++i) clk.lap(st, kClkLaps);
     }
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<kSLF, kProf, kHot>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline,
+    run_worker<kSLF, kProf, kHot, kExt>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline,
                                   kHot ? reinterpret_cast<HopBlock *>(s_blk[w]) : nullptr);
     if (kProf) st.v[kClkTotal] = clock64() - t_begin;
     flush_stats(a.q.ctl, st);
@@ -547,7 +547,7 @@ struct BatchArgs {
     uint64_t *qtime;           // per query: wall-clock ticks at start and end
 };
 
-template <int kWaves, bool kProf>
+template <int kWaves, bool kProf, bool kExt = false>
 __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
     __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
@@ -584,7 +584,8 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
         __threadfence();
         __syncthreads();
         a.dist = b.dist_all + (uint64_t)qi * a.n;
-        run_worker<true, kProf>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline);
+        run_worker<true, kProf, false, kExt>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st, clk,
+                                             deadline);
         __syncthreads();
         if (threadIdx.x == 0) {
             b.qtime[2 * qi + 1] = wall_clock64();
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(256) void serve_reset_kernel(ServeRequest *reqs, ui
     }
 }
 
-template <int kWaves>
+template <int kWaves, bool kExt = false>
 __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base, ServeRequest *reqs,
                                                                  uint32_t wg_per_req) {
     __shared__ uint64_t s_ring[kWaves][kL1Alloc];
@@ -664,7 +665,8 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base,
     for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
     PhaseClock<false> clk;
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<true, false>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline);
+    run_worker<true, false, false, kExt>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
+                                         clk, deadline);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -934,10 +936,17 @@ hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t
     b.qstatus = B.qstatus;
     b.qtime = B.qtime;
     const dim3 grid(L.grid_blocks);
-    if (B.waves == 16)
+    const bool ext = L.light_split || L.defer_heavy || L.strict_window;
+    if (B.waves == 16 && ext)
+        hipLaunchKernelGGL((sssp_batch_kernel<16, false, true>), grid, dim3(16 * kWave), 0, s, b);
+    else if (B.waves == 16)
         hipLaunchKernelGGL((sssp_batch_kernel<16, false>), grid, dim3(16 * kWave), 0, s, b);
+    else if (B.waves == 8 && ext)
+        hipLaunchKernelGGL((sssp_batch_kernel<8, false, true>), grid, dim3(8 * kWave), 0, s, b);
     else if (B.waves == 8)
         hipLaunchKernelGGL((sssp_batch_kernel<8, false>), grid, dim3(8 * kWave), 0, s, b);
+    else if (ext)
+        hipLaunchKernelGGL((sssp_batch_kernel<4, false, true>), grid, dim3(4 * kWave), 0, s, b);
     else
         hipLaunchKernelGGL((sssp_batch_kernel<4, false>), grid, dim3(4 * kWave), 0, s, b);
     return hipGetLastError();
@@ -960,10 +969,17 @@ hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t
     if (chunks > 512) chunks = 512;
     hipLaunchKernelGGL(serve_reset_kernel, dim3(chunks, S.nreq), dim3(256), 0, s, reqs, L.n, part_slots);
     const dim3 grid(S.nreq * S.wg_per_req);
-    if (S.waves == 16)
+    const bool ext = L.light_split || L.defer_heavy || L.strict_window;
+    if (S.waves == 16 && ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<16, true>), grid, dim3(16 * kWave), 0, s, a, reqs, S.wg_per_req);
+    else if (S.waves == 16)
         hipLaunchKernelGGL((sssp_serve_kernel<16>), grid, dim3(16 * kWave), 0, s, a, reqs, S.wg_per_req);
+    else if (S.waves == 8 && ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<8, true>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
     else if (S.waves == 8)
         hipLaunchKernelGGL((sssp_serve_kernel<8>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
+    else if (ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<4, true>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
     else
         hipLaunchKernelGGL((sssp_serve_kernel<4>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
     return hipGetLastError();
@@ -982,16 +998,25 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     const SolveArgs a = make_args(L);
     const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
     const bool hot = L.slf && L.hot;
+    const bool ext = L.light_split || L.defer_heavy || L.strict_window;
     if (L.slf && hot && !L.profile)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, true>), grid, block, 0, s, a);
     else if (L.slf && hot)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, true>), grid, block, 0, s, a);
+    else if (L.slf && !L.profile && ext)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, false, true>), grid, block, 0, s, a);
     else if (L.slf && !L.profile)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, false>), grid, block, 0, s, a);
+    else if (L.slf && ext)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, false, true>), grid, block, 0, s, a);
     else if (L.slf)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, false>), grid, block, 0, s, a);
+    else if (!L.profile && ext)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, false, false, true>), grid, block, 0, s, a);
     else if (!L.profile)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, false, false>), grid, block, 0, s, a);
+    else if (ext)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, true, false, true>), grid, block, 0, s, a);
     else
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, false, true, false>), grid, block, 0, s, a);
     return hipGetLastError();
