@@ -45,8 +45,20 @@ struct L2Queue {
     uint32_t strict;     // 1: the window advances only once every taken record is credited
 };
 
+// Per-wave statistics: event counters in 32 bits (a wave's share of one solve
+// stays far below 2^32), phase clocks (profiling build only) in 64 bits.
+// Both are wave-uniform and live in scalar registers; the narrow counters
+// keep the scalar register file from spilling.
 struct WaveStats {
-    uint64_t v[kNumStats];
+    uint32_t v[kNumStats];
+    uint64_t clk[kNumStats];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (uint32_t i = 0; i < kNumStats; ++i) {
+            v[i] = 0;
+            clk[i] = 0;
+        }
+    }
 };
 
 __device__ __forceinline__ uint64_t *slot_ptr(const L2Queue &q, uint32_t b, uint32_t pos) {

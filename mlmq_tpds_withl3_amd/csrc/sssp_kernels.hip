@@ -31,17 +31,24 @@ struct SolveArgs {
     uint32_t *dist;         // n distances
     L2Queue q;
     uint32_t n;
-    uint32_t min_read;
-    uint32_t max_sleep;     // idle back-off cap (x 256 cycles)
-    uint32_t pop_max;       // records taken from L1 per batch (<= 64)
-    uint32_t spill;         // L1 spill threshold (<= kL1Cap - 7 * 64)
-    uint32_t donate_min;    // donate to L2 while waves are idle and L1 holds more (0 = off)
-    uint32_t fifo;          // 1: L1 is a FIFO (queue mode L1V_L2DQ), 0: SLF deque
-    uint32_t split;         // 1: split hub edge lists into chunk records
-    uint32_t stale_check;   // 2: coherent dist load, 1: cached load, 0: none
-    uint32_t prefetch;      // 1: touch each relaxed child's hop block during the atomics
-    uint32_t defer_heavy;   // 1: heavy records beyond the head bucket go back to L2
-    uint32_t light_split;   // 1: relax light edges now, heavy edges (w >= delta) one bucket later
+    // Small settings packed two words, to spare scalar registers in the
+    // persistent loop: lims = min_read | pop_max << 8 | spill << 16 |
+    // max_sleep << 24 (each <= 64); flags = fifo | split << 1 |
+    // stale_check << 2 | prefetch << 4 | defer_heavy << 5 | light_split << 6 |
+    // donate_min << 8.
+    uint32_t lims;
+    uint32_t flags;
+    __device__ __forceinline__ uint32_t min_read() const { return lims & 0xFFu; }
+    __device__ __forceinline__ uint32_t pop_max() const { return (lims >> 8) & 0xFFu; }
+    __device__ __forceinline__ uint32_t spill() const { return (lims >> 16) & 0xFFu; }
+    __device__ __forceinline__ uint32_t max_sleep() const { return lims >> 24; }
+    __device__ __forceinline__ uint32_t fifo() const { return flags & 1u; }
+    __device__ __forceinline__ bool split() const { return (flags >> 1) & 1u; }
+    __device__ __forceinline__ uint32_t stale_check() const { return (flags >> 2) & 3u; }
+    __device__ __forceinline__ bool prefetch() const { return (flags >> 4) & 1u; }
+    __device__ __forceinline__ bool defer_heavy() const { return (flags >> 5) & 1u; }
+    __device__ __forceinline__ bool light_split() const { return (flags >> 6) & 1u; }
+    __device__ __forceinline__ uint32_t donate_min() const { return (flags >> 8) & 0xFFu; }
     uint64_t timeout_ticks;
 };
 
@@ -54,7 +61,7 @@ struct PhaseClock {
     __device__ __forceinline__ void lap(WaveStats &st, uint32_t slot) {
         if (kProf) {
             const uint64_t now = clock64();
-            st.v[slot] += now - t;
+            st.clk[slot] += now - t;
             t = now;
         }
     }
@@ -143,7 +150,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         st.v[kStatImprove] += sums >> 16;
         if (sums >> 16) {
             if (kSLF) {
-                ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill, a.fifo,
+                ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill(), a.fifo(),
                                                          a.q, st, blk);
             } else {
 #pragma unroll
@@ -202,7 +209,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t v = rec_vertex(rec);
     // Light/heavy split: a heavy-phase record relaxes only its vertex's edges
     // of weight >= delta (the reference has no such split; see SolveArgs).
-    const bool hphase = (kExt && a.light_split) && (rec_dist(rec) & kHeavyPhase) != 0;
+    const bool hphase = (kExt && a.light_split()) && (rec_dist(rec) & kHeavyPhase) != 0;
     const uint32_t d = rec_dist(rec) & ~kHeavyPhase;
     // A chunk record names (hub id, chunk index); one extra round trip to the
     // hub table, taken only by batches that hold chunks.
@@ -215,7 +222,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         }
     }
     // Stale filter (reference :174, process on <=).  Distances only fall, so
-    // a value read through this XCD's L2 (a.stale_check == 1) is never below
+    // a value read through this XCD's L2 (a.stale_check() == 1) is never below
     // the coherent one: the filter then lets some stale records through but
     // never drops a live one.  0 = no filter (every record is expanded).
     // Hot lanes (block-carrying L1) already hold their block: no round trip;
@@ -223,8 +230,8 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t cur = kUnreached;
     HopBlock b = empty_block();
     if (valid && !hot) {
-        if (a.stale_check == 2) cur = ld_relaxed(a.dist + v);
-        else if (a.stale_check == 1) cur = a.dist[v];
+        if (a.stale_check() == 2) cur = ld_relaxed(a.dist + v);
+        else if (a.stale_check() == 1) cur = a.dist[v];
         b = load_block(a.blocks, v);
     }
     if (kHot) {
@@ -238,15 +245,15 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         }
     }
     // idle-wave count for work donation, in the same round trip
-    hungry = a.donate_min ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
+    hungry = a.donate_min() ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
     // bucket epoch, for deferring heavy records (same round trip)
-    const uint32_t epoch = (kExt && a.defer_heavy) ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
+    const uint32_t epoch = (kExt && a.defer_heavy()) ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
     bool keep = valid && (hot || d <= cur);
     st.v[kStatStale] += popc(ballot(valid && !keep));
     // Deferral: a heavy record whose distance lies beyond the head bucket goes
     // back to L2 (into its own bucket) instead of being expanded now, so hub
     // edge lists are expanded in bucket order and rarely twice.
-    if (kExt && a.defer_heavy) {
+    if (kExt && a.defer_heavy()) {
         const bool defer = keep && !chunk && b.deg() > kInlineEdges && d / a.q.delta > epoch;
         if (ballot(defer)) {
             st.v[kStatSpare1] += popc(ballot(defer));
@@ -262,14 +269,14 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         const uint32_t lo = ci * csize;
         deg = lo < deg ? min(csize, deg - lo) : 0u;
         hstart += lo;
-    } else if (a.split && deg > kInlineEdges && hub_word != 0) {
+    } else if (a.split() && deg > kInlineEdges && hub_word != 0) {
         nextra = (deg - 1) / csize;
         deg = min(deg, csize);
     }
     // chunk lanes always take the CSR path (their edges are not inline)
     const bool csr_path = chunk || b.deg() > kInlineEdges;
     bool need_heavy = false;   // a light-phase record whose vertex has heavy edges
-    if ((kExt && a.light_split) && csr_path && !chunk) {
+    if ((kExt && a.light_split()) && csr_path && !chunk) {
         // adjacency sorted by weight: the first b.q[1].x edges are light
         const uint32_t nl = min(b.q[1].x, deg);
         if (hphase) {
@@ -295,7 +302,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         const uint32_t wk = word_of(b, 3 + 2 * k);
         nd[k] = d + wk;
         bool e = (uint32_t)k < ideg;
-        if ((kExt && a.light_split)) {
+        if ((kExt && a.light_split())) {
             const bool hv = wk >= a.q.delta;
             need_heavy = need_heavy || (e && hv && !hphase);
             e = e && hv == hphase;
@@ -316,7 +323,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 #pragma unroll
         for (int k = 0; k < (kHot ? (int)kInlineEdges : 0); ++k)
             cb[k] = load_block(a.blocks, ((on >> k) & 1) ? c[k] : 0u);
-    } else if (a.prefetch) {
+    } else if (a.prefetch()) {
 #pragma unroll
         for (int k = 0; k < (int)kInlineEdges; ++k) {
             const uint32_t t = ((on >> k) & 1) ? c[k] : 0u;
@@ -333,7 +340,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // every child to a strictly lower distance and pushes it, so the children
     // this stale expansion improved need not be pushed.
     uint32_t cur2 = 0;
-    if (kHot && hot && a.stale_check) cur2 = ld_relaxed(a.dist + v);
+    if (kHot && hot && a.stale_check()) cur2 = ld_relaxed(a.dist + v);
     // Keep every result opaque until all atomics are issued; otherwise the
     // compiler sinks each comparison into its predicated block and waits
     // for the atomics one at a time.
@@ -348,7 +355,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         out[k] = make_rec(c[k], nd[k]);
     }
     if (kHot) {
-        const bool late_stale = hot && a.stale_check && cur2 < d;
+        const bool late_stale = hot && a.stale_check() && cur2 < d;
         st.v[kStatStale] += popc(ballot(late_stale));
         if (late_stale) bits = 0;
     }
@@ -362,7 +369,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     bool ok = true;
     if (sums >> 16) {
         if (kSLF) {
-            ok = l1_push<kInlineEdges, kHot ? 1 : 0>(ring, l1, out, bits, a.spill, a.fifo, a.q,
+            ok = l1_push<kInlineEdges, kHot ? 1 : 0>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
                                                      st, blk, cb);
         } else {
 #pragma unroll
@@ -409,16 +416,16 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         bool hot = false;
         uint32_t hw[16];
         if (kHot) {
-            nin = l1_pop_hot(ring, blk, l1, a.pop_max, rec, hot, hw);
+            nin = l1_pop_hot(ring, blk, l1, a.pop_max(), rec, hot, hw);
             clk.lap(st, kClkL1Pop);
         } else if (kSLF) {
-            nin = l1_pop(ring, l1, a.pop_max, rec);
+            nin = l1_pop(ring, l1, a.pop_max(), rec);
             clk.lap(st, kClkL1Pop);
         }
         // Top up from L2 below min_read.  A block-carrying L1 may hold back
         // cold records behind a hot run; it tops up only once it is empty, so
         // a hot batch stays free of global loads.
-        if (nin < a.min_read && (!kHot || l1.size == 0)) {
+        if (nin < a.min_read() && (!kHot || l1.size == 0)) {
             const uint32_t got = l2_pop<kExt>(q, cl_start, cl_mask, stage, kWave - nin, st);
             if (got) {
                 credit += got;
@@ -432,7 +439,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
                 if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
                 credit = 0;
             }
-            if (a.donate_min && !idle_listed) {
+            if (a.donate_min() && !idle_listed) {
                 if (lane == 0) fetch_add(q.ctl + kHungryOff, 1u);
                 idle_listed = true;
             }
@@ -443,7 +450,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
                 raise_error(q, kErrTimeout);
                 break;
             }
-            sleep = sleep ? (sleep * 2 < a.max_sleep ? sleep * 2 : a.max_sleep) : 1;
+            sleep = sleep ? (sleep * 2 < a.max_sleep() ? sleep * 2 : a.max_sleep()) : 1;
             for (uint32_t k = 0; k < sleep; ++k) __builtin_amdgcn_s_sleep(4);
             clk.lap(st, kClkIdle);
             continue;
@@ -458,7 +465,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         if (!process_batch<kSLF, kProf, kHot, kExt>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
                                               st, clk))
             break;
-        if (kSLF && hungry && l1.size > a.donate_min) {
+        if (kSLF && hungry && l1.size > a.donate_min()) {
             // (uniform: hungry is a scalar load result)
             if (!l1_donate(ring, l1, uniform(l1.size) / 2, q, st)) break;
             clk.lap(st, kClkFlush);
@@ -491,8 +498,10 @@ __device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
     if (lane_id() == 0) {
         uint64_t *stats = reinterpret_cast<uint64_t *>(ctl + kStatsOff);
 #pragma unroll
-        for (uint32_t i = 0; i < kNumStats; ++i)
-            if (st.v[i]) add64(stats + i, st.v[i]);
+        for (uint32_t i = 0; i < kNumStats; ++i) {
+            const uint64_t x = (i >= kClkL1Pop && i <= kClkLaps) ? st.clk[i] : (uint64_t)st.v[i];
+            if (x) add64(stats + i, x);
+        }
     }
 }
 
@@ -510,8 +519,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     const uint32_t gwave = blockIdx.x * kWaves + w;
 
     WaveStats st;
-#pragma unroll
-    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    st.clear();
     PhaseClock<kProf> clk;
     const uint64_t t_begin = kProf ? clock64() : 0;
     clk.start();
@@ -521,7 +529,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
     run_worker<kSLF, kProf, kHot, kExt>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline,
                                   kHot ? reinterpret_cast<HopBlock *>(s_blk[w]) : nullptr);
-    if (kProf) st.v[kClkTotal] = clock64() - t_begin;
+    if (kProf) st.clk[kClkTotal] = clock64() - t_begin;
     flush_stats(a.q.ctl, st);
 }
 
@@ -560,8 +568,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
     a.q.slots = b.slots_all + ((uint64_t)blockIdx.x * a.q.nbuckets << a.q.cap_log2);
 
     WaveStats st;
-#pragma unroll
-    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    st.clear();
     PhaseClock<kProf> clk;
     clk.start();
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
@@ -661,8 +668,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base,
     if (part == 0 && threadIdx.x == 0) r->t_start = wall_clock64();
 
     WaveStats st;
-#pragma unroll
-    for (uint32_t i = 0; i < kNumStats; ++i) st.v[i] = 0;
+    st.clear();
     PhaseClock<false> clk;
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
     run_worker<true, false, false, kExt>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
@@ -901,17 +907,11 @@ static SolveArgs make_args(const SolverLaunch &L) {
     a.q.spin_limit = L.spin_limit;
     a.q.strict = L.strict_window;
     a.n = L.n;
-    a.min_read = L.min_read;
-    a.max_sleep = L.max_sleep;
-    a.pop_max = L.pop_max;
-    a.spill = L.spill;
-    a.donate_min = L.donate_min;
-    a.fifo = L.fifo ? 1u : 0u;
-    a.split = L.hubs && L.split ? 1u : 0u;
-    a.stale_check = L.stale_check;
-    a.prefetch = L.prefetch;
-    a.defer_heavy = L.defer_heavy;
-    a.light_split = L.light_split;
+    a.lims = (L.min_read & 0xFFu) | (L.pop_max & 0xFFu) << 8 | (L.spill & 0xFFu) << 16 |
+             (L.max_sleep & 0xFFu) << 24;
+    a.flags = (L.fifo ? 1u : 0u) | (L.hubs && L.split ? 2u : 0u) | (L.stale_check & 3u) << 2 |
+              (L.prefetch ? 16u : 0u) | (L.defer_heavy ? 32u : 0u) | (L.light_split ? 64u : 0u) |
+              (L.donate_min & 0xFFu) << 8;
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }
