@@ -83,6 +83,7 @@ __device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
 // kHeavyUnroll chunks of 64 slots are handled per iteration: all their
 // column/weight loads, then all their atomicMins, are in flight together.
 constexpr int kHeavyUnroll = 4;
+static_assert(kHeavyUnroll * 64 < 1024, "relax_heavy packs per-wave sums in 10-bit fields");
 
 template <bool kSLF, int kBlk>
 __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
@@ -129,11 +130,25 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
+        // Cached pre-check: a destination's distance read through this CU's
+        // and XCD's caches may be stale, but distances only fall, so a
+        // cached value <= nd proves the atomicMin could not lower it.  Most
+        // relaxations out of high-degree vertices fail, and skipping their
+        // atomics (each a trip to the memory-side atomic unit) is what
+        // bounds this loop; the check costs one cache round trip.
+        uint32_t pre[kHeavyUnroll];
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) pre[j] = act[j] ? a.dist[c[j]] : 0u;
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pre[j]));
         uint32_t pv[kHeavyUnroll];
+        uint32_t nat = 0;
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) {
             pv[j] = 0;
-            if (act[j]) pv[j] = fetch_min(a.dist + c[j], nd[j]);
+            const bool go = act[j] && nd[j] < pre[j];
+            nat += go ? 1u : 0u;
+            if (go) pv[j] = fetch_min(a.dist + c[j], nd[j]);
         }
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pv[j]));
@@ -145,10 +160,14 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
             nact += act[j] ? 1u : 0u;
             out[j] = make_rec(c[j], nd[j]);
         }
-        const uint32_t sums = wave_sum(nact | ((uint32_t)__builtin_popcount(bits) << 16));
-        st.v[kStatRelax] += sums & 0xFFFFu;
-        st.v[kStatImprove] += sums >> 16;
-        if (sums >> 16) {
+        // relaxations, skipped atomics and improvements: 10-bit fields
+        // (each at most kHeavyUnroll * 64 per wave)
+        const uint32_t sums = wave_sum(nact | (nact - nat) << 10 |
+                                       (uint32_t)__builtin_popcount(bits) << 20);
+        st.v[kStatRelax] += sums & 0x3FFu;
+        st.v[kStatAtomic] += (sums >> 10) & 0x3FFu;   // skipped; see flush_stats
+        st.v[kStatImprove] += sums >> 20;
+        if (sums >> 20) {
             if (kSLF) {
                 ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill(), a.fifo(),
                                                          a.q, st, blk);
@@ -494,7 +513,8 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
 }
 
 __device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
-    st.v[kStatAtomic] = st.v[kStatRelax];
+    // kStatAtomic counts the atomics the heavy loop's pre-check skipped
+    st.v[kStatAtomic] = st.v[kStatRelax] - st.v[kStatAtomic];
     if (lane_id() == 0) {
         uint64_t *stats = reinterpret_cast<uint64_t *>(ctl + kStatsOff);
 #pragma unroll
