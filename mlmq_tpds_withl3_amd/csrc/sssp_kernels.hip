@@ -77,13 +77,78 @@ __device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
     }
 }
 
-// Heavy records (more out-edges than a hop block holds): edge slots spread
-// over the lanes by a DPP scan of degrees, the owner of a slot found as a DPP
-// running maximum of start markers, columns and weights read from the CSR.
-// kHeavyUnroll chunks of 64 slots are handled per iteration: all their
-// column/weight loads, then all their atomicMins, are in flight together.
+// Heavy records (more out-edges than a hop block holds) read their columns
+// and weights from the CSR in two tiers:
+//   1. each full 64-edge piece of a record is relaxed by the whole wave, lane
+//      i taking edge i (coalesced loads).  The owner of piece j is the
+//      highest lane whose first piece index is <= j: one ballot and a
+//      find-last-bit, no LDS;
+//   2. the tails (deg mod 64 edges of each record) are spread over the lanes
+//      by a DPP scan of their lengths, the owner of a slot found as a DPP
+//      running maximum of start markers written to LDS.
+// Each iteration handles kHeavyUnroll groups of 64 slots: all their
+// column/weight loads, then all their pre-checks, then all their atomicMins,
+// are in flight together.
 constexpr int kHeavyUnroll = 4;
-static_assert(kHeavyUnroll * 64 < 1024, "relax_heavy packs per-wave sums in 10-bit fields");
+static_assert(kHeavyUnroll * 64 < 1024, "relax_slots packs per-wave sums in 10-bit fields");
+
+// Relax kHeavyUnroll slots per lane whose destinations c[] and tentative
+// distances nd[] are loaded; push the improved destinations.
+template <bool kSLF, int kBlk>
+__device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act)[kHeavyUnroll],
+                                            const uint32_t (&c)[kHeavyUnroll],
+                                            const uint32_t (&nd)[kHeavyUnroll], uint64_t *ring,
+                                            HopBlock *blk, L1State &l1, WaveStats &st) {
+    // Cached pre-check: a destination's distance read through this CU's and
+    // XCD's caches may be stale, but distances only fall, so a cached value
+    // <= nd proves the atomicMin could not lower it.  Most relaxations out of
+    // high-degree vertices fail, and skipping their atomics (each a trip to
+    // the memory-side atomic unit) is what bounds this loop; the check costs
+    // one cache round trip.
+    uint32_t pre[kHeavyUnroll];
+#pragma unroll
+    for (int j = 0; j < kHeavyUnroll; ++j) pre[j] = act[j] ? a.dist[c[j]] : 0u;
+#pragma unroll
+    for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pre[j]));
+    uint32_t pv[kHeavyUnroll];
+    uint32_t nat = 0;
+#pragma unroll
+    for (int j = 0; j < kHeavyUnroll; ++j) {
+        pv[j] = 0;
+        const bool go = act[j] && nd[j] < pre[j];
+        nat += go ? 1u : 0u;
+        if (go) pv[j] = fetch_min(a.dist + c[j], nd[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pv[j]));
+    uint64_t out[kHeavyUnroll];
+    uint32_t bits = 0, nact = 0;
+#pragma unroll
+    for (int j = 0; j < kHeavyUnroll; ++j) {
+        if (act[j] && nd[j] < pv[j]) bits |= 1u << j;
+        nact += act[j] ? 1u : 0u;
+        out[j] = make_rec(c[j], nd[j]);
+    }
+    // relaxations, skipped atomics and improvements: 10-bit fields (each at
+    // most kHeavyUnroll * 64 per wave)
+    const uint32_t sums =
+        wave_sum(nact | (nact - nat) << 10 | (uint32_t)__builtin_popcount(bits) << 20);
+    st.v[kStatRelax] += sums & 0x3FFu;
+    st.v[kStatAtomic] += (sums >> 10) & 0x3FFu;   // skipped; see flush_stats
+    st.v[kStatImprove] += sums >> 20;
+    bool ok = true;
+    if (sums >> 20) {
+        if (kSLF) {
+            ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill(), a.fifo(),
+                                                     a.q, st, blk);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kHeavyUnroll; ++j)
+                if (ok) ok = l2_push(a.q, out[j], (bits >> j) & 1, st);
+        }
+    }
+    return ok;
+}
 
 template <bool kSLF, int kBlk>
 __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
@@ -91,12 +156,45 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
                             uint32_t *map, WaveStats &st) {
     const uint32_t lane = lane_id();
     const uint32_t deg = heavy ? hdeg : 0u;
-    uint32_t total = 0;
-    const uint32_t excl = wave_excl_scan(deg, total);
-    // per lane: offset of its record's first edge relative to its first slot
-    const uint32_t off = hstart - excl;
-    uint32_t carry = 0;
     bool ok = true;
+
+    // tier 1: full 64-edge pieces, one per 64 slots
+    const uint32_t npc = deg / kWave;
+    uint32_t ptotal = 0;
+    const uint32_t pex = wave_excl_scan(npc, ptotal);   // first piece index per lane
+    for (uint32_t pb = 0; pb < ptotal && ok; pb += kHeavyUnroll) {
+        bool act[kHeavyUnroll];
+        uint32_t c[kHeavyUnroll], nd[kHeavyUnroll], o_d[kHeavyUnroll];
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) {
+            const uint32_t pj = pb + j;
+            act[j] = pj < ptotal;
+            const uint64_t m = ballot(npc > 0 && pex <= pj);
+            const uint32_t ow = m ? 63u - (uint32_t)__builtin_clzll(m) : 0u;
+            const uint32_t e = read_lane(hstart, ow) + (pj - read_lane(pex, ow)) * kWave + lane;
+            o_d[j] = read_lane(d, ow);
+            c[j] = 0;
+            nd[j] = 0;
+            if (act[j]) {
+                c[j] = a.col[e];
+                nd[j] = a.wgt[e];
+            }
+        }
+        // all column/weight loads issued before any is used
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
+#pragma unroll
+        for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
+        ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st);
+    }
+
+    // tier 2: the tails, spread over the lanes
+    const uint32_t tdeg = deg % kWave;
+    uint32_t total = 0;
+    const uint32_t excl = wave_excl_scan(tdeg, total);
+    // per lane: offset of its tail's first edge relative to its first slot
+    const uint32_t off = hstart + (deg - tdeg) - excl;
+    uint32_t carry = 0;
     for (uint32_t base = 0; base < total && ok; base += kHeavyUnroll * kWave) {
         uint32_t o[kHeavyUnroll];
         bool act[kHeavyUnroll];
@@ -106,7 +204,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
             act[j] = cb + lane < total;
             map[lane] = 0;
             wave_sync();
-            if (deg > 0 && excl >= cb && excl - cb < (uint32_t)kWave) map[excl - cb] = lane + 1;
+            if (tdeg > 0 && excl >= cb && excl - cb < (uint32_t)kWave) map[excl - cb] = lane + 1;
             wave_sync();
             const uint32_t o1 = max(carry, wave_incl_max(map[lane]));
             carry = read_lane(o1, kWave - 1);
@@ -125,58 +223,11 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
                 nd[j] = a.wgt[e];
             }
         }
-        // all column/weight loads issued before any is used
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
-        // Cached pre-check: a destination's distance read through this CU's
-        // and XCD's caches may be stale, but distances only fall, so a
-        // cached value <= nd proves the atomicMin could not lower it.  Most
-        // relaxations out of high-degree vertices fail, and skipping their
-        // atomics (each a trip to the memory-side atomic unit) is what
-        // bounds this loop; the check costs one cache round trip.
-        uint32_t pre[kHeavyUnroll];
-#pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) pre[j] = act[j] ? a.dist[c[j]] : 0u;
-#pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pre[j]));
-        uint32_t pv[kHeavyUnroll];
-        uint32_t nat = 0;
-#pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) {
-            pv[j] = 0;
-            const bool go = act[j] && nd[j] < pre[j];
-            nat += go ? 1u : 0u;
-            if (go) pv[j] = fetch_min(a.dist + c[j], nd[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pv[j]));
-        uint64_t out[kHeavyUnroll];
-        uint32_t bits = 0, nact = 0;
-#pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) {
-            if (act[j] && nd[j] < pv[j]) bits |= 1u << j;
-            nact += act[j] ? 1u : 0u;
-            out[j] = make_rec(c[j], nd[j]);
-        }
-        // relaxations, skipped atomics and improvements: 10-bit fields
-        // (each at most kHeavyUnroll * 64 per wave)
-        const uint32_t sums = wave_sum(nact | (nact - nat) << 10 |
-                                       (uint32_t)__builtin_popcount(bits) << 20);
-        st.v[kStatRelax] += sums & 0x3FFu;
-        st.v[kStatAtomic] += (sums >> 10) & 0x3FFu;   // skipped; see flush_stats
-        st.v[kStatImprove] += sums >> 20;
-        if (sums >> 20) {
-            if (kSLF) {
-                ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill(), a.fifo(),
-                                                         a.q, st, blk);
-            } else {
-#pragma unroll
-                for (int j = 0; j < kHeavyUnroll; ++j)
-                    if (ok) ok = l2_push(a.q, out[j], (bits >> j) & 1, st);
-            }
-        }
+        ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st);
     }
     return ok;
 }

@@ -141,6 +141,23 @@ def test_hub_chunk_boundaries(solver):
             solve_check(solver, row, col, w, src, delta=500)
 
 
+@pytest.mark.parametrize("cfg", [
+    dict(),
+    dict(queue="L1N_L2DQ"),
+    dict(light_split=1, delta=300),
+])
+def test_heavy_piece_boundaries(solver, cfg):
+    """Out-degrees at and around multiples of 64: full 64-edge pieces are
+    relaxed wave-wide, the remainders through the per-slot owner scan."""
+    say(f"heavy piece boundaries {cfg}")
+    degs = [8, 63, 64, 65, 127, 128, 129, 191, 192, 193, 640, 641, 1023, 4096, 4097] * 4
+    row, col, w = hub_csr(20000, degs, 40000, seed=11)
+    params = dict(delta=500)
+    params.update(cfg)
+    for src in (0, 17, 19999):
+        solve_check(solver, row, col, w, src, **params)
+
+
 def test_zero_weight_components(solver):
     say("zero weights")
     row, col, w = random_csr(5000, 15000, 99, wmax=3, zero_frac=0.6)
