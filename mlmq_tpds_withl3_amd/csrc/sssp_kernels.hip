@@ -379,12 +379,6 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         }
         if (e) on |= 1u << k;
     }
-    // Prefetch, in the same round trip as the atomics: touch every relaxed
-    // child's hop block, so the batch that later pops the child finds the
-    // line in this XCD's L2 (its dist line is already there from the
-    // atomic).  Unpredicated -- lanes without an edge touch vertex 0's
-    // block -- so there is no branch and no wait; the loaded words are
-    // consumed one batch later.
     // Block-carrying L1: load every relaxed child's whole block in this
     // round trip; the improved children take it into L1 with them, and the
     // batch that pops them needs no global load.
@@ -393,17 +387,26 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 #pragma unroll
         for (int k = 0; k < (kHot ? (int)kInlineEdges : 0); ++k)
             cb[k] = load_block(a.blocks, ((on >> k) & 1) ? c[k] : 0u);
-    } else if (a.prefetch()) {
-#pragma unroll
-        for (int k = 0; k < (int)kInlineEdges; ++k) {
-            const uint32_t t = ((on >> k) & 1) ? c[k] : 0u;
-            pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
-        }
     }
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         pv[k] = 0;
         if ((on >> k) & 1) pv[k] = fetch_min(a.dist + c[k], nd[k]);
+    }
+    // Prefetch, in the same round trip as the atomics: touch every relaxed
+    // child's hop block, so the batch that later pops the child finds the
+    // line in this XCD's L2 (its dist line is already there from the
+    // atomic).  Issued after the atomics and without a branch (lanes without
+    // an edge, and all lanes when a.prefetch() is off, touch vertex 0's
+    // block), so waiting for the atomics' results takes a fixed vmcnt that
+    // leaves these loads in flight.  The loaded words are consumed one batch
+    // later.
+    if (!kHot) {
+#pragma unroll
+        for (int k = 0; k < (int)kInlineEdges; ++k) {
+            const uint32_t t = (a.prefetch() && ((on >> k) & 1)) ? c[k] : 0u;
+            pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
+        }
     }
     // Hot lanes' stale check, in the same round trip.  A record whose vertex
     // has since dropped below d has a fresher record queued; that one relaxes
@@ -470,7 +473,14 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
     const uint32_t lane = lane_id();
     const L2Queue &q = a.q;
     L1State l1{0u, 0u};
-    uint32_t credit = 0;        // records taken from L2 whose work is not finished
+    // Records taken from L2 whose work is not finished.  Credits are returned
+    // with a relaxed atomic add: termination needs every tail reservation a
+    // record's expansion made to be performed before its credit, and each
+    // reservation's result (the slot index) was waited for before the
+    // record's children were written, so program order already gives that.
+    // A release would also write back this XCD's L2 and wait for the
+    // in-flight prefetch loads.
+    uint32_t credit = 0;
     uint32_t cl_start = 0;      // lane b: start of the claim on bucket b
     uint64_t cl_mask = 0;       // lane b: pending slots of that claim
     uint32_t sleep = 0;
@@ -506,7 +516,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         if (nin == 0) {
             if (credit) {
-                if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
+                if (lane == 0) fetch_add(q.ctl + kDoneOff, credit);
                 credit = 0;
             }
             if (a.donate_min() && !idle_listed) {
@@ -546,7 +556,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
             pf_prev[k] = pf[k];
         }
         if (credit && l1.size == 0) {
-            if (lane == 0) fetch_add_release(q.ctl + kDoneOff, credit);
+            if (lane == 0) fetch_add(q.ctl + kDoneOff, credit);
             credit = 0;
             clk.lap(st, kClkCredit);
         }
