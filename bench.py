@@ -282,7 +282,8 @@ def run_batch(args):
     sources = np.random.default_rng(args.source_seed).integers(0, g.n, size=args.queries)
     delta = args.delta if args.delta is not None else spec["delta"]
     base = dict(delta=delta, batch_waves=args.batch_waves, work_blocks=args.work_blocks,
-                l1_pop=args.l1_pop)
+                l1_pop=args.l1_pop, min_read=args.min_read, l2_batch=args.l2_batch,
+                buckets=args.buckets, bucket_window=args.window)
     solver = Solver(0)
     solver.load_csr(row, col, w)
     tuned = None

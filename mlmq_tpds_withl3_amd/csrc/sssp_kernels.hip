@@ -269,7 +269,12 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 //      back (up to 7 per lane) before any result is used.
 // A (dst, nd) record is pushed only if its atomicMin strictly lowered dst's
 // distance (:211-243).  The batch's pushes go to L1 in one pass.
-template <bool kSLF, bool kProf, bool kHot, bool kExt>
+// kPre puts a cached pre-check of every destination's distance before its
+// atomicMin (a third round trip, see relax_slots): the many-query kernels,
+// which keep the whole device busy, are bound by the memory-side atomic
+// traffic and gain from it; the single-query kernel, bound by the latency
+// of its critical path, does not.
+template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
                               const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
                               uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &hungry,
@@ -388,10 +393,21 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         for (int k = 0; k < (kHot ? (int)kInlineEdges : 0); ++k)
             cb[k] = load_block(a.blocks, ((on >> k) & 1) ? c[k] : 0u);
     }
+    uint32_t go = on;   // bit k: edge k issues its atomicMin
+    if (kPre) {
+        uint32_t pre[kInlineEdges];
+#pragma unroll
+        for (int k = 0; k < (int)kInlineEdges; ++k) pre[k] = ((on >> k) & 1) ? a.dist[c[k]] : 0u;
+#pragma unroll
+        for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pre[k]));
+#pragma unroll
+        for (int k = 0; k < (int)kInlineEdges; ++k)
+            if (nd[k] >= pre[k]) go &= ~(1u << k);
+    }
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
         pv[k] = 0;
-        if ((on >> k) & 1) pv[k] = fetch_min(a.dist + c[k], nd[k]);
+        if ((go >> k) & 1) pv[k] = fetch_min(a.dist + c[k], nd[k]);
     }
     // Prefetch, in the same round trip as the atomics: touch every relaxed
     // child's hop block, so the batch that later pops the child finds the
@@ -424,7 +440,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t bits = 0;   // bit k: edge k lowered its destination's distance
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) {
-        if (((on >> k) & 1) && nd[k] < pv[k]) bits |= 1u << k;
+        if (((go >> k) & 1) && nd[k] < pv[k]) bits |= 1u << k;
         out[k] = make_rec(c[k], nd[k]);
     }
     if (kHot) {
@@ -466,7 +482,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 // loop, SS/SSSP/sssp_run.cu:507-584).  Returns when the query's exit flag is
 // seen, after an error, or at the deadline.  `leader` waves also run the
 // termination check while idle.
-template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false>
+template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false, bool kPre = false>
 __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
                            uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
                            uint64_t deadline, HopBlock *blk = nullptr) {
@@ -542,7 +558,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf, kHot, kExt>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
+        if (!process_batch<kSLF, kProf, kHot, kExt, kPre>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
                                               st, clk))
             break;
         if (kSLF && hungry && l1.size > a.donate_min()) {
@@ -672,7 +688,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
         __threadfence();
         __syncthreads();
         a.dist = b.dist_all + (uint64_t)qi * a.n;
-        run_worker<true, kProf, false, kExt>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st, clk,
+        run_worker<true, kProf, false, kExt, true>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st, clk,
                                              deadline);
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -752,7 +768,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base,
     st.clear();
     PhaseClock<false> clk;
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<true, false, false, kExt>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
+    run_worker<true, false, false, kExt, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
                                          clk, deadline);
     __threadfence();
     __syncthreads();
