@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--mode", choices=["single", "batch", "serve"], default="single",
                     help="single: one query per step (default); batch: all sources in one launch")
     ap.add_argument("--queries", type=int, default=256, help="batch mode: number of sources")
-    ap.add_argument("--rates", default="100,250,500,1000,2000,5000",
+    ap.add_argument("--rates", default="100,250,500,1000,2000,3000,4000,5000",
                     help="serve mode: Poisson arrival rates (queries/s)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve mode: seconds per rate")
     ap.add_argument("--serve-graphs", default="road264k,road6m,rmat20")

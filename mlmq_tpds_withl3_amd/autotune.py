@@ -106,6 +106,8 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
 
     cur = dict(base or {})
     cur.setdefault("delta", 200000)
+    cur.setdefault("prefetch", 1)
+    cur.setdefault("stale_check", 2)
 
     def score(params):
         p = default_params(timeout_s=60.0, **params)
@@ -119,7 +121,10 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
 
     best_t = score(cur)
     trail = [(dict(cur), best_t)]
-    space = [("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
+    # prefetch and the stale-check mode first: with the whole device busy the
+    # child-block touches cost more memory-side bandwidth than they save
+    space = [("prefetch", [0, 1]), ("stale_check", [1, 2]),
+             ("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
              ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32])]
     for key, values in space:
         for v in values:
