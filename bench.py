@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--buckets", type=int, default=16)
     ap.add_argument("--window", type=int, default=4)
     ap.add_argument("--light-split", type=int, default=0)
+    ap.add_argument("--stale-check", type=int, default=2,
+                    help="2 coherent, 1 through the L2, 3 late (in the atomics' round trip), 0 none")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -125,7 +127,7 @@ def main():
                               min_read=args.min_read, buckets=args.buckets,
                               bucket_window=args.window, l1_pop=args.l1_pop,
                               l1_donate=args.l1_donate, light_split=args.light_split,
-                              timeout_s=30.0)
+                              stale_check=args.stale_check, timeout_s=30.0)
 
     solver = Solver(0)
     solver.load_csr(row, col, w)
@@ -149,6 +151,7 @@ def main():
         args.buckets = best.get("buckets", args.buckets)
         args.window = best.get("bucket_window", args.window)
         args.light_split = best.get("light_split", args.light_split)
+        args.stale_check = best.get("stale_check", args.stale_check)
         params = make_params(delta)
         log(f"tuned over delta {list(spec['delta_sweep'])}: {best} {tuned['device_ms']:.3f} ms")
         if args.extended_sweep:
@@ -222,6 +225,7 @@ def main():
             "delta_sweep_device_ms": {str(k): round(v, 4) for k, v in sweep.items()},
             "outside_sweep_best": extended,
             "buckets": args.buckets, "bucket_window": args.window, "light_split": args.light_split,
+            "stale_check": args.stale_check,
             "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop, "l1_donate": args.l1_donate,
             "work_blocks": last["work_blocks"], "waves": last["waves"],
             "cus": shape["cus"], "blocks_per_cu_max": shape["blocks_per_cu_max"],

@@ -29,8 +29,9 @@ def default_delta(row, w) -> int:
 def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None) -> dict:
     """Grid search for single queries: every delta in `deltas` crossed with
     `grid` (a list of parameter dicts, default: L1 pop size / L2 top-up
-    threshold pairs and two grid sizes).  Each point is the best device time
-    of `repeats` solves.  Returns the best point and the whole table."""
+    threshold pairs and two grid sizes), then the late and L2-cached stale
+    checks at the four best points.  Each point is the best device time of
+    `repeats` solves.  Returns the best point and the whole table."""
     from .solver import default_params
 
     if grid is None:
@@ -39,14 +40,22 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None)
                 for wb in (64, 128) for nb in (16, 4, 2)
                 for pop, mr, lb in ((32, 32, 8), (32, 32, 32), (32, 16, 16), (32, 8, 8), (64, 8, 8))]
     table = []
+
+    def measure(params):
+        p = default_params(timeout_s=30.0, **params)
+        ms = min(solver.solve(src, p)["device_ms"] for _ in range(repeats))
+        table.append((params, ms))
+        if log:
+            log(f"tune single: {params} -> {ms:.3f} ms")
+
     for d in deltas:
         for g in grid:
-            params = dict(g, delta=int(d))
-            p = default_params(timeout_s=30.0, **params)
-            ms = min(solver.solve(src, p)["device_ms"] for _ in range(repeats))
-            table.append((params, ms))
-            if log:
-                log(f"tune single: {params} -> {ms:.3f} ms")
+            measure(dict(g, delta=int(d)))
+    # second stage: the other stale-check modes at the four best points
+    for bp, _ in sorted(table, key=lambda t: t[1])[:4]:
+        for sc in (3, 1):
+            if bp.get("stale_check", 2) != sc:
+                measure(dict(bp, stale_check=sc))
     best = min(table, key=lambda t: t[1])
     return {"params": best[0], "device_ms": best[1], "table": table}
 

@@ -570,7 +570,7 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
     if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
     if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
-    if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
+    if (p.stale_check < 0 || p.stale_check > 3) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1, 2 or 3");
     if (p.l1_blocks != 0 && p.l1_blocks != 1) return fail(c, MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.light_split && (p.hub_split || p.defer_heavy))
@@ -745,7 +745,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
     if (p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
         return fail(c, MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
-    if (p.stale_check < 0 || p.stale_check > 2) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1 or 2");
+    if (p.stale_check < 0 || p.stale_check > 3) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1, 2 or 3");
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.light_split && (p.hub_split || p.defer_heavy))
         return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");

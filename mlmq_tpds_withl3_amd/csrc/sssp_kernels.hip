@@ -300,8 +300,10 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // a value read through this XCD's L2 (a.stale_check() == 1) is never below
     // the coherent one: the filter then lets some stale records through but
     // never drops a live one.  0 = no filter (every record is expanded).
-    // Hot lanes (block-carrying L1) already hold their block: no round trip;
-    // their stale check is made after the atomics (see below).
+    // 3 = late: the distance is read in the atomics' round trip instead (see
+    // below), so this round trip waits only for the block.  Hot lanes
+    // (block-carrying L1) already hold their block: no round trip; their
+    // stale check is always late.
     uint32_t cur = kUnreached;
     HopBlock b = empty_block();
     if (valid && !hot) {
@@ -362,7 +364,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             deg = nl;
         }
     }
-    const bool heavy = csr_path && deg > 0;
+    bool heavy = csr_path && deg > 0;
     const uint32_t ideg = csr_path ? 0u : deg;
     clk.lap(st, kClkRow);
     if (ballot(nextra > 0)) {
@@ -409,6 +411,15 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         pv[k] = 0;
         if ((go >> k) & 1) pv[k] = fetch_min(a.dist + c[k], nd[k]);
     }
+    // Late stale check, in the same round trip (issued before the touches
+    // below, so waiting for it leaves them in flight).  A record whose vertex
+    // has since dropped below d has a fresher record queued; that one relaxes
+    // every child to a strictly lower distance and pushes it, so the children
+    // this stale expansion improved need not be pushed, and its CSR edges
+    // need not be relaxed at all.
+    const bool late_chk = (kHot && hot && a.stale_check()) || (valid && a.stale_check() == 3);
+    uint32_t cur2 = kUnreached;
+    if (late_chk) cur2 = ld_relaxed(a.dist + v);
     // Prefetch, in the same round trip as the atomics: touch every relaxed
     // child's hop block, so the batch that later pops the child finds the
     // line in this XCD's L2 (its dist line is already there from the
@@ -424,18 +435,12 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
         }
     }
-    // Hot lanes' stale check, in the same round trip.  A record whose vertex
-    // has since dropped below d has a fresher record queued; that one relaxes
-    // every child to a strictly lower distance and pushes it, so the children
-    // this stale expansion improved need not be pushed.
-    uint32_t cur2 = 0;
-    if (kHot && hot && a.stale_check()) cur2 = ld_relaxed(a.dist + v);
     // Keep every result opaque until all atomics are issued; otherwise the
     // compiler sinks each comparison into its predicated block and waits
     // for the atomics one at a time.
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pv[k]));
-    if (kHot) asm volatile("" : "+v"(cur2));
+    asm volatile("" : "+v"(cur2));
     uint64_t out[kInlineEdges];
     uint32_t bits = 0;   // bit k: edge k lowered its destination's distance
 #pragma unroll
@@ -443,10 +448,14 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         if (((go >> k) & 1) && nd[k] < pv[k]) bits |= 1u << k;
         out[k] = make_rec(c[k], nd[k]);
     }
-    if (kHot) {
-        const bool late_stale = hot && a.stale_check() && cur2 < d;
+    {
+        const bool late_stale = late_chk && cur2 < d;
         st.v[kStatStale] += popc(ballot(late_stale));
-        if (late_stale) bits = 0;
+        if (late_stale) {
+            bits = 0;
+            heavy = false;
+            need_heavy = false;
+        }
     }
     // relaxations and improvements of the batch in one wave reduction
     const uint32_t sums = wave_sum((uint32_t)__builtin_popcount(on) |

@@ -76,7 +76,7 @@ def test_hub_batch(solver, split):
     check_batch(solver, row, col, w, sources, work_blocks=3, hub_split=split, delta=300)
 
 
-@pytest.mark.parametrize("stale", [0, 1])
+@pytest.mark.parametrize("stale", [0, 1, 3])
 def test_stale_modes_batch(solver, stale):
     say(f"stale_check={stale} batch")
     row, col, w = grid_csr(70, 80, 5)

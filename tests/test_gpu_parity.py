@@ -82,6 +82,8 @@ def test_random_graphs(solver, seed, queue):
     dict(queue="L1V_L2DQ", l1_spill=8, l1_pop=16),               # FIFO spilling often
     dict(stale_check=1, delta=2000),                             # stale filter through the L2
     dict(stale_check=0, delta=2000),                             # no stale filter
+    dict(stale_check=3, delta=2000),                             # late stale filter
+    dict(stale_check=3, queue="L1N_L2DQ", delta=500),
     dict(prefetch=0, delta=2000),
     dict(l1_blocks=1, delta=2000),                               # block-carrying L1
     dict(l1_blocks=1, queue="L1V_L2DQ"),
@@ -117,6 +119,8 @@ HUBS = [70000, 9000, 4097, 2049, 2048, 1500]   # degrees 70002 .. 1504: five hub
     dict(light_split=1, delta=100, hub_split=0),
     dict(light_split=1, delta=20, buckets=64, bucket_window=4, hub_split=0),
     dict(light_split=1, delta=500, queue="L1V_L2DQ", hub_split=0),
+    dict(stale_check=3),                            # late check skips stale chunks' edges
+    dict(stale_check=3, light_split=1, delta=100, hub_split=0),
 ])
 def test_hub_split(solver, cfg):
     say(f"hubs {cfg}")
@@ -145,6 +149,7 @@ def test_hub_chunk_boundaries(solver):
     dict(),
     dict(queue="L1N_L2DQ"),
     dict(light_split=1, delta=300),
+    dict(stale_check=3, delta=200),
 ])
 def test_heavy_piece_boundaries(solver, cfg):
     """Out-degrees at and around multiples of 64: full 64-edge pieces are
@@ -170,7 +175,7 @@ def test_road264k(solver):
     g, src, spec = make_workload("road264k")
     row, col, w = g.arrays()
     for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ"),
-                dict(stale_check=1), dict(stale_check=0), dict(l1_blocks=1),
+                dict(stale_check=1), dict(stale_check=0), dict(stale_check=3), dict(l1_blocks=1),
                 dict(light_split=1, delta=20000)):
         say(f"road264k {cfg}")
         t, work, reached = solve_check(solver, row, col, w, src, **cfg)
@@ -241,6 +246,9 @@ def test_errors_and_recovery(solver):
     assert e.value.code == -1
     with pytest.raises(MLMQError) as e:
         solver.solve(0, buckets=0)
+    assert e.value.code == -1
+    with pytest.raises(MLMQError) as e:
+        solver.solve(0, stale_check=4)
     assert e.value.code == -1
     with pytest.raises(MLMQError) as e:
         solver.solve(0, queue_mode=3)   # L1V_L2V: not on this path
