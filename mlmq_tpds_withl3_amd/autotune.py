@@ -132,7 +132,7 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
     trail = [(dict(cur), best_t)]
     # prefetch and the stale-check mode first: with the whole device busy the
     # child-block touches cost more memory-side bandwidth than they save
-    space = [("prefetch", [0, 1]), ("stale_check", [1, 2]),
+    space = [("prefetch", [0, 1]), ("stale_check", [1, 3, 2]),
              ("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
              ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32])]
     for key, values in space:
