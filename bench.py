@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--buckets", type=int, default=16)
     ap.add_argument("--window", type=int, default=4)
     ap.add_argument("--light-split", type=int, default=0)
+    ap.add_argument("--prefetch", type=int, default=1,
+                    help="1: touch relaxed children's hop blocks in the atomics' round trip")
     ap.add_argument("--stale-check", type=int, default=2,
                     help="2 coherent, 1 through the L2, 3 late (in the atomics' round trip), 0 none")
     ap.add_argument("--cpu-runs", type=int, default=3)
@@ -127,7 +129,8 @@ def main():
                               min_read=args.min_read, buckets=args.buckets,
                               bucket_window=args.window, l1_pop=args.l1_pop,
                               l1_donate=args.l1_donate, light_split=args.light_split,
-                              stale_check=args.stale_check, timeout_s=30.0)
+                              stale_check=args.stale_check, prefetch=args.prefetch,
+                              timeout_s=30.0)
 
     solver = Solver(0)
     solver.load_csr(row, col, w)
@@ -287,7 +290,8 @@ def run_batch(args):
     delta = args.delta if args.delta is not None else spec["delta"]
     base = dict(delta=delta, batch_waves=args.batch_waves, work_blocks=args.work_blocks,
                 l1_pop=args.l1_pop, min_read=args.min_read, l2_batch=args.l2_batch,
-                buckets=args.buckets, bucket_window=args.window)
+                buckets=args.buckets, bucket_window=args.window, prefetch=args.prefetch,
+                stale_check=args.stale_check)
     solver = Solver(0)
     solver.load_csr(row, col, w)
     tuned = None

@@ -130,11 +130,13 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
 
     best_t = score(cur)
     trail = [(dict(cur), best_t)]
-    # prefetch and the stale-check mode first: with the whole device busy the
-    # child-block touches cost more memory-side bandwidth than they save
-    space = [("prefetch", [0, 1]), ("stale_check", [1, 3, 2]),
-             ("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
-             ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32])]
+    # Shape first, then the child-block touches and the stale-check mode
+    # (with the whole device busy the touches can cost more memory-side
+    # bandwidth than they save), then the queue sizes once more.
+    space = [("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
+             ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32]),
+             ("prefetch", [0, 1]), ("stale_check", [1, 3, 2]),
+             ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
     for key, values in space:
         for v in values:
             if cur.get(key) == v:
