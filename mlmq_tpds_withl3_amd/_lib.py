@@ -137,7 +137,8 @@ class ServeGraphParams(ctypes.Structure):
         ("l1_spill", ctypes.c_int32), ("workgroups", ctypes.c_int32),
         ("waves", ctypes.c_int32), ("bucket_capacity", ctypes.c_int64),
         ("hub_split", ctypes.c_int32), ("light_split", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("prefetch", ctypes.c_int32), ("stale_check", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 

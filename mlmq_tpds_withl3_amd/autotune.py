@@ -88,6 +88,10 @@ def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int
              ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
     if not cur.get("light_split"):
         space.append(("buckets", [16, 4, 2]))
+    # child-block touches and the stale-check mode last (-1 = no touches)
+    cur.setdefault("prefetch", 1)
+    cur.setdefault("stale_check", 2)
+    space += [("prefetch", [-1, 1]), ("stale_check", [1, 3, 2])]
     for key, values in space:
         for v in values:
             if cur.get(key) == v:

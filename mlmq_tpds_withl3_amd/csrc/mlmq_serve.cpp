@@ -124,6 +124,9 @@ int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n,
     if (p.hub_split != 0 && p.hub_split != 1) return fail(s, MLMQ_E_INVALID, "hub_split must be 0 or 1");
     if (p.light_split != 0 && p.light_split != 1) return fail(s, MLMQ_E_INVALID, "light_split must be 0 or 1");
     if (p.light_split && p.hub_split) return fail(s, MLMQ_E_INVALID, "light_split excludes hub_split");
+    if (p.prefetch < -1 || p.prefetch > 1) return fail(s, MLMQ_E_INVALID, "prefetch must be -1, 0 or 1");
+    if (p.stale_check < -1 || p.stale_check > 3)
+        return fail(s, MLMQ_E_INVALID, "stale_check must be -1, 0, 1, 2 or 3");
     uint64_t cap = (uint64_t)p.bucket_capacity;
     if (cap == 0) cap = std::max<uint64_t>((uint64_t)n / 32, 8192);
     const uint32_t c = ceil_log2(cap);
@@ -491,8 +494,8 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.blocks = g.v.blocks;
         L.hubs = g.v.hubs;
         L.split = g.p.hub_split != 0;
-        L.stale_check = 2;
-        L.prefetch = 1;
+        L.stale_check = g.p.stale_check == 0 ? 2u : g.p.stale_check < 0 ? 0u : (uint32_t)g.p.stale_check;
+        L.prefetch = g.p.prefetch >= 0 ? 1u : 0u;
         L.hot = false;
         L.defer_heavy = 0;
         L.strict_window = 0;

@@ -90,7 +90,7 @@ def test_ctypes_layouts_match_c(tmp_path):
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
         "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "reserved"]),
         "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "light_split",
-                                                              "reserved"]),
+                                                              "prefetch", "stale_check", "reserved"]),
         "mlmq_serve_request": (_lib.ServeRequest, ["src", "dist_out"]),
         "mlmq_serve_result": (_lib.ServeResult, ["status", "retries"]),
         "mlmq_serve_stats": (_lib.ServeStats, ["requests", "peak_inflight", "reserved"]),

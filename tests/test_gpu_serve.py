@@ -167,6 +167,19 @@ def test_workgroups_per_request(graphs, waves, wgs):
     s.close()
 
 
+@pytest.mark.parametrize("prm", [dict(prefetch=-1), dict(stale_check=1), dict(stale_check=3),
+                                 dict(stale_check=-1, prefetch=-1)])
+def test_prefetch_and_stale_modes_served(graphs, prm):
+    say(f"serve {prm}")
+    s, ids = new_server(graphs, streams=2, max_batch=8)
+    for gi in range(3):
+        s.set_graph_params(gi, delta=2000 if gi < 2 else 200000, workgroups=2, **prm)
+    gsel = np.array([0, 1, 2, 2, 1, 0, 2], dtype=np.int32)
+    src = np.array([3, 40, 11, 19000, 4000, 2999, 0], dtype=np.int32)
+    run_checked(s, graphs, np.linspace(0, 0.005, 7), gsel, src, f"{prm}")
+    s.close()
+
+
 def test_hub_graph_served(graphs):
     say("hub graph served")
     g = G(*hub_csr(20000, [40000, 5000, 2100], 40000, seed=8))
@@ -212,6 +225,10 @@ def test_argument_errors(graphs):
         s.run([0.0], [0], [graphs[0].n])            # source out of range
     with pytest.raises(MLMQError):
         s.set_graph_params(0, waves=6)
+    with pytest.raises(MLMQError):
+        s.set_graph_params(0, stale_check=4)
+    with pytest.raises(MLMQError):
+        s.set_graph_params(0, prefetch=2)
     # the server still works after refused calls
     run_checked(s, graphs, [0.0], [0], [1], "after errors")
     s.close()
