@@ -88,26 +88,34 @@ __device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
 //      running maximum of start markers written to LDS.
 // Each iteration handles kHeavyUnroll groups of 64 slots: all their
 // column/weight loads, then all their pre-checks, then all their atomicMins,
-// are in flight together.
+// are in flight together.  Tier 1 also loads the next iteration's columns
+// and weights while this one's pre-checks and atomics are in flight, so an
+// iteration costs two round trips instead of three.
 constexpr int kHeavyUnroll = 4;
 static_assert(kHeavyUnroll * 64 < 1024, "relax_slots packs per-wave sums in 10-bit fields");
 
-// Relax kHeavyUnroll slots per lane whose destinations c[] and tentative
-// distances nd[] are loaded; push the improved destinations.
-template <bool kSLF, int kBlk>
+// Relax kHeavyUnroll slots per lane whose destinations c[] (valid vertex
+// ids on every lane, active or not) and tentative distances nd[] are loaded;
+// push the improved destinations.  `mid()` runs after the pre-check loads are
+// issued and before they are waited for; any loads it issues stay in flight.
+template <bool kSLF, int kBlk, typename Mid>
 __device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act)[kHeavyUnroll],
                                             const uint32_t (&c)[kHeavyUnroll],
                                             const uint32_t (&nd)[kHeavyUnroll], uint64_t *ring,
-                                            HopBlock *blk, L1State &l1, WaveStats &st) {
+                                            HopBlock *blk, L1State &l1, WaveStats &st,
+                                            Mid &&mid) {
     // Cached pre-check: a destination's distance read through this CU's and
     // XCD's caches may be stale, but distances only fall, so a cached value
     // <= nd proves the atomicMin could not lower it.  Most relaxations out of
     // high-degree vertices fail, and skipping their atomics (each a trip to
     // the memory-side atomic unit) is what bounds this loop; the check costs
     // one cache round trip.
+    // (unpredicated, so the number of loads in flight is fixed and the wait
+    // below leaves mid()'s loads alone)
     uint32_t pre[kHeavyUnroll];
 #pragma unroll
-    for (int j = 0; j < kHeavyUnroll; ++j) pre[j] = act[j] ? a.dist[c[j]] : 0u;
+    for (int j = 0; j < kHeavyUnroll; ++j) pre[j] = a.dist[c[j]];
+    mid();
 #pragma unroll
     for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(pre[j]));
     uint32_t pv[kHeavyUnroll];
@@ -162,30 +170,49 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
     const uint32_t npc = deg / kWave;
     uint32_t ptotal = 0;
     const uint32_t pex = wave_excl_scan(npc, ptotal);   // first piece index per lane
-    for (uint32_t pb = 0; pb < ptotal && ok; pb += kHeavyUnroll) {
-        bool act[kHeavyUnroll];
-        uint32_t c[kHeavyUnroll], nd[kHeavyUnroll], o_d[kHeavyUnroll];
+    // Issue the column/weight loads of pieces [pb, pb + kHeavyUnroll).
+    // Unpredicated: pieces past the end read edge 0 (there is one) and are
+    // marked inactive.
+    auto load_pieces = [&](uint32_t pb, bool (&act)[kHeavyUnroll], uint32_t (&c)[kHeavyUnroll],
+                           uint32_t (&w)[kHeavyUnroll], uint32_t (&o_d)[kHeavyUnroll]) {
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) {
             const uint32_t pj = pb + j;
             act[j] = pj < ptotal;
             const uint64_t m = ballot(npc > 0 && pex <= pj);
             const uint32_t ow = m ? 63u - (uint32_t)__builtin_clzll(m) : 0u;
-            const uint32_t e = read_lane(hstart, ow) + (pj - read_lane(pex, ow)) * kWave + lane;
+            const uint32_t e =
+                act[j] ? read_lane(hstart, ow) + (pj - read_lane(pex, ow)) * kWave + lane : 0u;
             o_d[j] = read_lane(d, ow);
-            c[j] = 0;
-            nd[j] = 0;
-            if (act[j]) {
-                c[j] = a.col[e];
-                nd[j] = a.wgt[e];
-            }
+            c[j] = a.col[e];
+            w[j] = a.wgt[e];
         }
-        // all column/weight loads issued before any is used
+    };
+    if (ptotal) {
+        bool act[kHeavyUnroll];
+        uint32_t c[kHeavyUnroll], w[kHeavyUnroll], o_d[kHeavyUnroll];
+        load_pieces(0, act, c, w, o_d);
+        for (uint32_t pb = 0;;) {
 #pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
+            for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(w[j]));
+            uint32_t nd[kHeavyUnroll];
 #pragma unroll
-        for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
-        ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st);
+            for (int j = 0; j < kHeavyUnroll; ++j) nd[j] = w[j] + o_d[j];
+            const uint32_t nb = pb + kHeavyUnroll;
+            bool act2[kHeavyUnroll];
+            uint32_t c2[kHeavyUnroll], w2[kHeavyUnroll], o_d2[kHeavyUnroll];
+            ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st,
+                                         [&]() { load_pieces(nb, act2, c2, w2, o_d2); });
+            if (!ok || nb >= ptotal) break;
+#pragma unroll
+            for (int j = 0; j < kHeavyUnroll; ++j) {
+                act[j] = act2[j];
+                c[j] = c2[j];
+                w[j] = w2[j];
+                o_d[j] = o_d2[j];
+            }
+            pb = nb;
+        }
     }
 
     // tier 2: the tails, spread over the lanes
@@ -227,7 +254,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
-        ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st);
+        ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st, []() {});
     }
     return ok;
 }
