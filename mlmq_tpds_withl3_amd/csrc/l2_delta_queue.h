@@ -87,24 +87,36 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
         b = (base + (uint32_t)rel) % q.nbuckets;
     }
     st.v[kStatL2Push] += popc(todo);
-    bool ok = true;
-    while (todo) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+    // Group the lanes by bucket (register work only), then every group's
+    // leader reserves the group's slots: one fetch_add per bucket, all in one
+    // round trip, however many buckets the records span.
+    uint32_t rank = 0, cnt = 0, leader_of = 0;
+    for (uint64_t rest = todo; rest;) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(rest);
         const uint32_t bl = read_lane(b, leader);
-        const bool in_grp = ((todo >> lane) & 1) && b == bl;
+        const bool in_grp = ((rest >> lane) & 1) && b == bl;
         const uint64_t grp = ballot(in_grp);
-        uint32_t pos0 = 0;
-        if (lane == leader) pos0 = fetch_add(q.ctl + kTailOff + bl * kStride, popc(grp));
-        pos0 = read_lane(pos0, leader);
         if (in_grp) {
-            uint64_t *slot = slot_ptr(q, bl, pos0 + popc(grp & lanemask_lt()));
-            uint32_t spins = 0;
-            while (!cas64(slot, kEmpty, rec)) {
-                if (++spins > q.spin_limit) { ok = false; break; }
-                __builtin_amdgcn_s_sleep(2);
-            }
+            rank = popc(grp & lanemask_lt());
+            leader_of = leader;
         }
-        todo &= ~grp;
+        if (lane == leader) cnt = popc(grp);
+        rest &= ~grp;
+    }
+    uint32_t pos0 = 0;
+    if (cnt) pos0 = fetch_add(q.ctl + kTailOff + b * kStride, cnt);
+    pos0 = (uint32_t)__shfl((int)pos0, (int)leader_of, kWave);
+    bool ok = true;
+    if (valid) {
+        uint64_t *slot = slot_ptr(q, b, pos0 + rank);
+        uint32_t spins = 0;
+        while (!cas64(slot, kEmpty, rec)) {
+            if (++spins > q.spin_limit) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
     }
     if (ballot(!ok)) {
         raise_error(q, kErrOverflow);
