@@ -128,22 +128,33 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
 // Pop up to one claim's worth (at most `room`) of ready records into
 // `stage[0..got)`.  `cl_start`/`cl_mask` carry the wave's claims: lane b
 // holds bucket b.
+// Queues with more buckets than this read, per pop, only the counters of
+// the window and one bucket past it, placed by the epoch the wave saw last
+// (`base_hint`): every wave polls these lines, so reading 3 x 64 of them per
+// pop made the counters a hot spot.  The other buckets are read only when the
+// head bucket is drained and the read ones are empty (the window-advance
+// decision needs to know whether any bucket still holds records).
+constexpr uint32_t kNarrowQueue = 8;
+
 template <bool kStrict = false>
 __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
-                           uint64_t *stage, uint32_t room, WaveStats &st) {
+                           uint64_t *stage, uint32_t room, WaveStats &st, uint32_t &base_hint) {
     const uint32_t lane = lane_id();
     const bool mine = lane < q.nbuckets;
-    // one round trip: every bucket's counters and the epoch (issued last so
+    const bool wide = q.nbuckets > kNarrowQueue && !(kStrict && q.strict);
+    const bool rd = mine && (!wide || (lane + q.nbuckets - base_hint % q.nbuckets) % q.nbuckets <=
+                                          q.window);
+    // one round trip: the buckets' counters and the epoch (issued last so
     // that waiting for it does not hold back the others)
     uint32_t t = 0, h = 0, k = 0;
-    if (mine) {
+    if (rd) {
         t = ld_relaxed(q.ctl + kTailOff + lane * kStride);
         h = ld_relaxed(q.ctl + kHeadOff + lane * kStride);
         k = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
     }
     const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
     const uint32_t done = kStrict && q.strict ? uniform(ld_relaxed(q.ctl + kDoneOff)) : 0u;
-    const bool unclaimed = mine && (int32_t)(t - h) > 0;
+    const bool unclaimed = rd && (int32_t)(t - h) > 0;
     const uint64_t avail_mask = ballot(unclaimed);
     const uint64_t claim_mask = ballot(mine && cl_mask != 0);
     uint32_t got = 0;
@@ -186,13 +197,23 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
     // spawned, finished), so no record of the head bucket can still lower a
     // distance in the next one: a delta-stepping phase barrier.
     const uint32_t b0 = base % q.nbuckets;
-    const uint32_t live = mine ? t - k : 0;
-    if (read_lane(live, b0) == 0 && !((claim_mask >> b0) & 1)) {
-        const uint32_t total = uniform(wave_sum(live));
+    const uint32_t live = rd ? t - k : 0;
+    const bool b0_read = read_lane(rd ? 1u : 0u, b0) != 0;
+    if (b0_read && read_lane(live, b0) == 0 && !((claim_mask >> b0) & 1)) {
+        uint32_t total = uniform(wave_sum(live));
+        if (wide && total == 0) {
+            uint32_t t2 = 0, k2 = 0;
+            if (mine && !rd) {
+                t2 = ld_relaxed(q.ctl + kTailOff + lane * kStride);
+                k2 = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
+            }
+            total = uniform(wave_sum(t2 - k2));
+        }
         bool advance = total > 0;
         if (kStrict && q.strict) advance = advance && done == uniform(wave_sum(mine ? k : 0u));
         if (advance && lane == 0) cas32(q.ctl + kBaseOff, base, base + 1);
     }
+    base_hint = base;
     st.v[kStatL2Pop] += got;
     wave_sync();
     return got;

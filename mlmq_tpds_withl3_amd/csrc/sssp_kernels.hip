@@ -538,6 +538,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
     uint32_t sleep = 0;
     uint32_t pf[kInlineEdges], pf_prev[kInlineEdges];   // prefetch results, consumed one batch late
     uint32_t sunk = 0;
+    uint32_t base_hint = 0;     // L2 epoch seen by this wave's last pop
 #pragma unroll
     for (int k = 0; k < (int)kInlineEdges; ++k) pf[k] = pf_prev[k] = 0;
     bool idle_listed = false;   // counted in the control block's hungry-wave count
@@ -558,7 +559,8 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         // cold records behind a hot run; it tops up only once it is empty, so
         // a hot batch stays free of global loads.
         if (nin < a.min_read() && (!kHot || l1.size == 0)) {
-            const uint32_t got = l2_pop<kExt>(q, cl_start, cl_mask, stage, kWave - nin, st);
+            const uint32_t got =
+                l2_pop<kExt>(q, cl_start, cl_mask, stage, kWave - nin, st, base_hint);
             if (got) {
                 credit += got;
                 if (lane >= nin && lane < nin + got) rec = stage[lane - nin];
