@@ -66,8 +66,10 @@ typedef struct mlmq_ctx mlmq_ctx;
 typedef struct mlmq_params {
     int32_t  queue_mode;      /* MLMQ_QUEUE_L1SLF_L2DQ (default), _L1V_L2DQ or _L1N_L2DQ */
     uint32_t delta;           /* bucket width; reference BENCH_DELTA, default 200000 */
-    int32_t  work_blocks;     /* persistent workgroups; 0 = automatic (blocks_per_cu x CUs if
-                                 blocks_per_cu > 0, else n / 100000 clamped to [32, 128]) */
+    int32_t  work_blocks;     /* persistent workgroups of one wave64 each; 0 = automatic
+                                 (blocks_per_cu x CUs if blocks_per_cu > 0, else n / 25000
+                                 clamped to [128, 512]); mlmq_solve_batch: workgroups of
+                                 batch_waves waves, one query each */
     int32_t  blocks_per_cu;   /* used when work_blocks == 0 */
     int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
     int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */

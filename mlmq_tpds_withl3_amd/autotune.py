@@ -37,7 +37,7 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None)
     if grid is None:
         grid = [dict(l1_pop=pop, min_read=mr, l2_batch=lb, work_blocks=wb, buckets=nb,
                      bucket_window=min(4, nb))
-                for wb in (64, 128) for nb in (16, 4, 2)
+                for wb in (256, 512) for nb in (16, 4, 2)
                 for pop, mr, lb in ((32, 32, 8), (32, 32, 32), (32, 16, 16), (32, 8, 8), (64, 8, 8))]
     table = []
 

@@ -596,11 +596,12 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     } else if (blocks == 0) {
         // Latency-bound solver: a road-like graph of n vertices keeps only
         // O(sqrt(n)) records in flight, so more waves mostly add queue
-        // polling.  Measured optimum on MI355X: ~32 workgroups (128 waves)
-        // at 0.26 M vertices, ~64 at 6.3 M.
-        blocks = c->n / 100000;
-        if (blocks < 32) blocks = 32;
-        if (blocks > 128) blocks = 128;
+        // polling.  Measured optimum on MI355X: ~256 waves at 0.26 M
+        // vertices, 512 at 6.3 M.
+        int waves = (int)(c->n / 25000);
+        if (waves < 128) waves = 128;
+        if (waves > 512) waves = 512;
+        blocks = waves / mlmq::kWavesPerBlock;
     }
     if (blocks > occ * c->cus) blocks = occ * c->cus;
 

@@ -131,14 +131,14 @@ WORKLOADS = {
                    delta_sweep=(50, 100, 200, 1_000, 10_000),
                    tune_grid=[dict(light_split=ls, work_blocks=wb, buckets=64 if ls else 2,
                                    bucket_window=2, l2_batch=b, min_read=b, l1_pop=b)
-                              for ls in (1, 0) for wb in (64, 128) for b in (32, 64)]),
+                              for ls in (1, 0) for wb in (256, 512) for b in (32, 64)]),
     # RMAT scale 20 (1,048,576 vertices), edge factor 16, weights [1, 1000] (serving mix)
     "rmat20": dict(kind="rmat", scale=20, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=5,
                    wmin=1, wmax=1000, source="maxdeg", delta=200_000,
                    delta_sweep=(50, 100, 200, 1_000, 10_000),
                    tune_grid=[dict(light_split=ls, work_blocks=wb, buckets=64 if ls else 2,
                                    bucket_window=2, l2_batch=b, min_read=b, l1_pop=b)
-                              for ls in (1, 0) for wb in (64, 128) for b in (32, 64)]),
+                              for ls in (1, 0) for wb in (256, 512) for b in (32, 64)]),
     # 1,070,376 points, 3-NN, longest edge -> 50,000 (batched-query graph)
     "road1m": dict(kind="knn", n=1_070_376, k=3, seed=4, weight_scale=-50_000.0, morton=True,
                    source="centroid", delta=200_000, delta_sweep=(100_000, 200_000, 400_000)),
