@@ -70,7 +70,7 @@ typedef struct mlmq_serve_graph_params {
     int32_t  l1_pop;          /* 1..64; 0 = 32 */
     int32_t  l1_spill;        /* 1..64; 0 = 64 */
     int32_t  workgroups;      /* workgroups per request, 1..64; 0 = auto from the edge count */
-    int32_t  waves;           /* waves per workgroup: 4, 8 or 16; 0 = 8 */
+    int32_t  waves;           /* waves per workgroup: 1, 4, 8 or 16; 0 = 8 */
     int64_t  bucket_capacity; /* ring slots per bucket (rounded up to 2^k); 0 = n / 32, >= 8192.
                                  Doubled automatically when a request overflows (the request
                                  is then run again). */

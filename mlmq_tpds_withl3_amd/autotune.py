@@ -84,8 +84,8 @@ def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int
 
     best_s = score(cur)
     trail = [(dict(cur), best_s)]
-    space = [("delta", _delta_grid(cur["delta"])), ("workgroups", [1, 2, 4, 8]),
-             ("waves", [4, 8, 16]), ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
+    space = [("delta", _delta_grid(cur["delta"])), ("workgroups", [1, 2, 4, 8, 16, 32]),
+             ("waves", [1, 4, 8, 16]), ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
     if not cur.get("light_split"):
         space.append(("buckets", [16, 4, 2]))
     # child-block touches and the stale-check mode last (-1 = no touches)

@@ -119,7 +119,8 @@ int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n,
     if (p.l1_pop < 1 || p.l1_pop > 64) return fail(s, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
     if (p.l1_spill < 1 || p.l1_spill > 64) return fail(s, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
     if (p.workgroups < 1 || p.workgroups > 64) return fail(s, MLMQ_E_INVALID, "workgroups must be in [1, 64]");
-    if (p.waves != 4 && p.waves != 8 && p.waves != 16) return fail(s, MLMQ_E_INVALID, "waves must be 4, 8 or 16");
+    if (p.waves != 1 && p.waves != 4 && p.waves != 8 && p.waves != 16)
+        return fail(s, MLMQ_E_INVALID, "waves must be 1, 4, 8 or 16");
     if (p.bucket_capacity < 0) return fail(s, MLMQ_E_INVALID, "negative bucket_capacity");
     if (p.hub_split != 0 && p.hub_split != 1) return fail(s, MLMQ_E_INVALID, "hub_split must be 0 or 1");
     if (p.light_split != 0 && p.light_split != 1) return fail(s, MLMQ_E_INVALID, "light_split must be 0 or 1");

@@ -1088,9 +1088,10 @@ hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t
 }
 
 hipError_t serve_occupancy(int waves, int *blocks_per_cu) {
-    const void *fn = waves == 16 ? reinterpret_cast<const void *>(&sssp_serve_kernel<16>)
+    const void *fn = waves == 16  ? reinterpret_cast<const void *>(&sssp_serve_kernel<16>)
                      : waves == 8 ? reinterpret_cast<const void *>(&sssp_serve_kernel<8>)
-                                  : reinterpret_cast<const void *>(&sssp_serve_kernel<4>);
+                     : waves == 4 ? reinterpret_cast<const void *>(&sssp_serve_kernel<4>)
+                                  : reinterpret_cast<const void *>(&sssp_serve_kernel<1>);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
 }
 
@@ -1113,10 +1114,14 @@ hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t
         hipLaunchKernelGGL((sssp_serve_kernel<8, true>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
     else if (S.waves == 8)
         hipLaunchKernelGGL((sssp_serve_kernel<8>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
-    else if (ext)
+    else if (S.waves == 4 && ext)
         hipLaunchKernelGGL((sssp_serve_kernel<4, true>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
-    else
+    else if (S.waves == 4)
         hipLaunchKernelGGL((sssp_serve_kernel<4>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
+    else if (ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<1, true>), grid, dim3(kWave), 0, s, a, reqs, S.wg_per_req);
+    else
+        hipLaunchKernelGGL((sssp_serve_kernel<1>), grid, dim3(kWave), 0, s, a, reqs, S.wg_per_req);
     return hipGetLastError();
 }
 

@@ -155,7 +155,7 @@ def test_overflow_reruns(graphs):
     s.close()
 
 
-@pytest.mark.parametrize("waves,wgs", [(4, 1), (8, 3), (16, 2), (4, 8)])
+@pytest.mark.parametrize("waves,wgs", [(4, 1), (8, 3), (16, 2), (4, 8), (1, 1), (1, 5), (1, 32)])
 def test_workgroups_per_request(graphs, waves, wgs):
     say(f"waves {waves} workgroups {wgs}")
     s, ids = new_server(graphs, streams=3, max_batch=8)
