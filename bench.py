@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--pool", type=int, default=16, help="serve mode: verified sources per graph")
     ap.add_argument("--budget-gib", type=float, default=64.0)
     ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--spread-waves", type=int, default=512,
+                    help="serve mode: waves a launch of few requests may spread over")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--max-pending", type=int, default=4096)
     ap.add_argument("--p99-bound-ms", type=float, default=1000.0,
@@ -374,6 +376,7 @@ def run_serve(args):
         raise SystemExit("--mix needs one weight per graph")
     rng = np.random.default_rng(args.source_seed)
     server = Server(0, hbm_budget=int(args.budget_gib * (1 << 30)), streams=args.streams,
+                    spread_waves=args.spread_waves,
                     max_batch=args.max_batch, max_pending=args.max_pending, verify=1,
                     timeout_s=60.0)
     graphs, pools, seq_ms, tuned = [], [], [], []
@@ -455,6 +458,7 @@ def run_serve(args):
         "config": {"workload": "serve:" + "+".join(names), "mode": "serve", "mix": mix,
                    "duration_s": args.duration, "pool_per_graph": args.pool,
                    "budget_gib": args.budget_gib, "streams": args.streams,
+                   "spread_waves": args.spread_waves,
                    "max_batch": args.max_batch, "max_pending": args.max_pending,
                    "p99_bound_ms": args.p99_bound_ms, "graphs": graphs},
         "best_rate": best["rate"] if best else None,

@@ -125,7 +125,7 @@ class ServerConfig(ctypes.Structure):
         ("hbm_budget", ctypes.c_uint64), ("streams", ctypes.c_int32),
         ("max_batch", ctypes.c_int32), ("max_pending", ctypes.c_int32),
         ("verify", ctypes.c_int32), ("timeout_s", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 8),
+        ("spread_waves", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
     ]
 
 

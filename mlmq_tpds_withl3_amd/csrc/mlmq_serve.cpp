@@ -226,6 +226,7 @@ void mlmq_server_config_default(mlmq_server_config *cfg) {
     cfg->max_pending = 0;
     cfg->verify = 1;
     cfg->timeout_s = 60.0;
+    cfg->spread_waves = 512;
 }
 
 void mlmq_serve_graph_params_default(mlmq_serve_graph_params *p) {
@@ -253,7 +254,8 @@ int mlmq_server_create(int device, const mlmq_server_config *cfg, mlmq_server **
     if (c.streams == 0) c.streams = 4;
     if (c.max_batch == 0) c.max_batch = 64;
     if (c.timeout_s <= 0) c.timeout_s = 60.0;
-    if (c.streams < 1 || c.streams > 32 || c.max_batch < 1 || c.max_batch > 4096 || c.max_pending < 0)
+    if (c.streams < 1 || c.streams > 32 || c.max_batch < 1 || c.max_batch > 4096 || c.max_pending < 0 ||
+        c.spread_waves < 0 || c.spread_waves > 65536)
         return MLMQ_E_INVALID;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return MLMQ_E_INVALID;
@@ -514,14 +516,21 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.spill = (uint32_t)g.p.l1_spill;
         L.donate_min = 0;
         L.timeout_ticks = timeout_ticks;
-        L.grid_blocks = (int)(nb * (uint32_t)g.p.workgroups);
+        // Few requests in the launch: give each more workgroups, up to
+        // spread_waves waves for the whole launch.
+        uint32_t wg = (uint32_t)g.p.workgroups;
+        if (s->cfg.spread_waves > 0) {
+            const uint32_t fill = (uint32_t)s->cfg.spread_waves / (nb * (uint32_t)g.p.waves);
+            if (fill > wg) wg = std::min<uint32_t>(fill, 64u);
+        }
+        L.grid_blocks = (int)(nb * wg);
         L.slf = true;
         L.fifo = false;
         L.profile = false;
         mlmq::ServeLaunch S;
         S.reqs = l.d_reqs;
         S.nreq = nb;
-        S.wg_per_req = (uint32_t)g.p.workgroups;
+        S.wg_per_req = wg;
         S.waves = g.p.waves;
         HIP_TRY(s, mlmq::launch_serve(L, S, l.s));
         HIP_TRY(s, hipEventRecord(l.ev_solve, l.s));

@@ -88,7 +88,7 @@ def test_ctypes_layouts_match_c(tmp_path):
         "mlmq_timing": (_lib.Timing, ["donated", "hub_chunks"]),
         "mlmq_batch_timing": (_lib.BatchTiming, None),
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
-        "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "reserved"]),
+        "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "spread_waves", "reserved"]),
         "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "light_split",
                                                               "prefetch", "stale_check", "reserved"]),
         "mlmq_serve_request": (_lib.ServeRequest, ["src", "dist_out"]),

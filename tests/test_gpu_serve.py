@@ -180,6 +180,18 @@ def test_prefetch_and_stale_modes_served(graphs, prm):
     s.close()
 
 
+@pytest.mark.parametrize("spread", [0, 64, 4096])
+def test_spread_waves(graphs, spread):
+    """Launches of few requests spread them over more workgroups (up to 64
+    per request); results do not depend on it."""
+    say(f"spread_waves {spread}")
+    s, ids = new_server(graphs, streams=2, max_batch=4, spread_waves=spread)
+    gsel = np.array([0, 1, 2, 2, 1, 0, 2], dtype=np.int32)
+    src = np.array([3, 40, 11, 19000, 4000, 2999, 0], dtype=np.int32)
+    run_checked(s, graphs, np.linspace(0, 0.02, 7), gsel, src, f"spread {spread}")
+    s.close()
+
+
 def test_hub_graph_served(graphs):
     say("hub graph served")
     g = G(*hub_csr(20000, [40000, 5000, 2100], 40000, seed=8))
