@@ -57,10 +57,11 @@ typedef struct mlmq_server_config {
     int32_t  max_pending;     /* waiting requests before arrivals are rejected; 0 = no limit */
     int32_t  verify;          /* compare results with registered expected rows */
     double   timeout_s;       /* device watchdog per launch; <= 0 means 60 s */
-    int32_t  spread_waves;    /* a launch of few requests gives them up to this many waves in
-                                 total (more workgroups per request than the graph's setting,
-                                 at most 64), so a request arriving at a quiet server gets much
-                                 of the device; 0 = off (default 512) */
+    int32_t  spread_waves;    /* a launch of few requests runs them on up to this many waves in
+                                 total, as one-wave workgroups (at most 512 per request), when
+                                 that is more than the graph's workgroups x waves: a request
+                                 arriving at a quiet server gets much of the device; 0 = off
+                                 (default 512) */
     int32_t  reserved[7];
 } mlmq_server_config;
 

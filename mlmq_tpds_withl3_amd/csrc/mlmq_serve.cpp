@@ -78,6 +78,10 @@ struct mlmq_server {
 
 namespace {
 
+// Most one-wave workgroups a spread request gets (the single-query solver's
+// best wave count on the largest road graph).
+constexpr uint32_t kMaxSpread = 512;
+
 int fail(mlmq_server *s, int code, const std::string &msg) {
     if (s) s->err = msg;
     return code;
@@ -516,12 +520,16 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         L.spill = (uint32_t)g.p.l1_spill;
         L.donate_min = 0;
         L.timeout_ticks = timeout_ticks;
-        // Few requests in the launch: give each more workgroups, up to
-        // spread_waves waves for the whole launch.
-        uint32_t wg = (uint32_t)g.p.workgroups;
+        // Few requests in the launch: spread each over more waves, up to
+        // spread_waves for the whole launch, as one-wave workgroups (they go
+        // to distinct CUs, as in the single-query solver).
+        uint32_t wg = (uint32_t)g.p.workgroups, waves = (uint32_t)g.p.waves;
         if (s->cfg.spread_waves > 0) {
-            const uint32_t fill = (uint32_t)s->cfg.spread_waves / (nb * (uint32_t)g.p.waves);
-            if (fill > wg) wg = std::min<uint32_t>(fill, 64u);
+            const uint32_t fill = std::min<uint32_t>((uint32_t)s->cfg.spread_waves / nb, kMaxSpread);
+            if (fill > wg * waves) {
+                wg = fill;
+                waves = 1;
+            }
         }
         L.grid_blocks = (int)(nb * wg);
         L.slf = true;
@@ -531,7 +539,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         S.reqs = l.d_reqs;
         S.nreq = nb;
         S.wg_per_req = wg;
-        S.waves = g.p.waves;
+        S.waves = (int)waves;
         HIP_TRY(s, mlmq::launch_serve(L, S, l.s));
         HIP_TRY(s, hipEventRecord(l.ev_solve, l.s));
         if (any_expect) HIP_TRY(s, mlmq::launch_serve_verify(S, L.n, l.s));

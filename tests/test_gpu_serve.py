@@ -146,7 +146,8 @@ def test_rejection_when_queue_full(graphs):
 
 def test_overflow_reruns(graphs):
     say("overflow reruns")
-    s, ids = new_server(graphs[2:], streams=2, max_batch=4)
+    # no spreading: the request's few workgroups must push through the tiny L2
+    s, ids = new_server(graphs[2:], streams=2, max_batch=4, spread_waves=0)
     s.set_graph_params(0, delta=2000, buckets=4, bucket_window=2, bucket_capacity=64)
     g = graphs[2]
     src = np.array([5, 777, 15000], dtype=np.int32)
