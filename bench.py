@@ -409,7 +409,7 @@ def run_serve(args):
                       "light_split")
         base = {k: v for k, v in single.items() if k in serve_keys}
         if args.tune:
-            t = tune_serving(server, gid, pool, base=base, log=log)
+            t = tune_serving(server, gid, pool, base=base, repeats=2, log=log)
             tuned.append(t["params"])
             log(f"serve graph {gid} tuned: {t['params']} {t['qps']:.1f} q/s")
         else:
