@@ -1,0 +1,114 @@
+"""Host logic of the parameter search (no GPU): the tuners, driven by fake
+solvers whose cost functions have a known best point, must return that point
+and explore the options they promise to explore."""
+import math
+import time
+
+import numpy as np
+
+from mlmq_tpds_withl3_amd import autotune as A
+
+
+class FakeSolver:
+    """solve(): device time from a cost function of the parameters;
+    solve_batch(): sleeps for three times the cost in ms (the batch tuner
+    times wall clock; the factor keeps sleep jitter below the cost steps)."""
+
+    def __init__(self, cost):
+        self.cost = cost
+        self.seen = []
+
+    @staticmethod
+    def params_dict(p):
+        return {k: getattr(p, k) for k, _ in p._fields_ if k != "reserved"}
+
+    def solve(self, src, p):
+        d = self.params_dict(p)
+        self.seen.append(d)
+        return {"device_ms": self.cost(d)}
+
+    def solve_batch(self, sources, p):
+        d = self.params_dict(p)
+        self.seen.append(d)
+        time.sleep(3 * self.cost(d) / 1e3)
+        return {}
+
+
+def single_cost(d):
+    # best: delta 100000, 512 waves, 4 buckets, pop 32 / top-up 16 / claim 16, late stale check
+    c = 5.0 + abs(math.log2(d["delta"] / 100000))
+    c += 0.0 if d["work_blocks"] == 512 else 0.5
+    c += 0.0 if d["buckets"] == 4 else 0.3
+    c += 0.0 if (d["l1_pop"], d["min_read"], d["l2_batch"]) == (32, 16, 16) else 0.2
+    c += {3: 0.0, 2: 0.1, 1: 0.4}.get(d["stale_check"], 1.0)
+    return c
+
+
+def test_tune_single_finds_best_and_tries_stale_modes():
+    s = FakeSolver(single_cost)
+    r = A.tune_single(s, 0, (50000, 100000, 200000), repeats=1)
+    best = r["params"]
+    assert best["delta"] == 100000 and best["work_blocks"] == 512 and best["buckets"] == 4
+    assert (best["l1_pop"], best["min_read"], best["l2_batch"]) == (32, 16, 16)
+    assert best["stale_check"] == 3
+    assert math.isclose(r["device_ms"], 5.0)
+    # the second stage measured both other stale-check modes at the best grid point
+    at_best = [d for d in s.seen if d["delta"] == 100000 and d["work_blocks"] == 512
+               and d["buckets"] == 4 and d["l1_pop"] == 32 and d["min_read"] == 16]
+    assert {d["stale_check"] for d in at_best} == {1, 2, 3}
+    # every table entry is one measured point
+    assert len(r["table"]) == len(s.seen)
+
+
+def test_tune_single_custom_grid():
+    s = FakeSolver(lambda d: 1.0 + d["light_split"] * -0.5 + d["work_blocks"] / 1e4)
+    grid = [dict(light_split=ls, work_blocks=wb) for ls in (0, 1) for wb in (256, 512)]
+    r = A.tune_single(s, 0, (50,), grid=grid, repeats=1)
+    assert r["params"]["light_split"] == 1 and r["params"]["work_blocks"] == 256
+
+
+def test_tune_batch_descends_every_coordinate():
+    def cost(d):
+        c = 2.0 + 0.5 * abs(math.log2(d["delta"] / 200000))
+        c += {16: 0.0, 8: 0.5, 4: 1.0}[d["batch_waves"]]
+        c += 0.0 if d["prefetch"] == 0 else 0.5
+        c += 0.0 if d["stale_check"] == 1 else 0.5
+        c += 0.0 if d["min_read"] == 16 else 0.3
+        return c
+
+    s = FakeSolver(cost)
+    r = A.tune_batch(s, np.arange(4), base=dict(delta=200000, batch_waves=8, min_read=8,
+                                                 l2_batch=8, l1_pop=32, buckets=16))
+    p = r["params"]
+    assert p["batch_waves"] == 16 and p["prefetch"] == 0 and p["stale_check"] == 1
+    assert p["min_read"] == 16 and p["l2_batch"] == 16 and p["delta"] == 200000
+    assert r["batch_s"] < 0.03
+
+
+class FakeServer:
+    def __init__(self, qps):
+        self.qps = qps
+        self.params = {}
+        self.calls = 0
+
+    def set_graph_params(self, gid, **params):
+        self.params = dict(params)
+
+    def run(self, arrival, graph, src):
+        self.calls += 1
+        n = len(arrival)
+        return None, {"completed": n, "mismatched": 0, "completed_qps": self.qps(self.params)}
+
+
+def test_tune_serving_picks_highest_rate_and_applies_it():
+    def qps(p):
+        q = 1000.0 - 100 * abs(math.log2(p.get("delta", 200000) / 50000))
+        q += 200 if p.get("workgroups") == 16 else 0
+        q += 100 if p.get("prefetch") == -1 else 0
+        return q
+
+    srv = FakeServer(qps)
+    r = A.tune_serving(srv, 0, np.arange(10), base=dict(delta=200000), repeats=1)
+    assert r["params"]["delta"] == 50000 and r["params"]["workgroups"] == 16
+    assert r["params"]["prefetch"] == -1
+    assert srv.params == r["params"]          # the best point is left applied
