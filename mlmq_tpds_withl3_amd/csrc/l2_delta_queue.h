@@ -23,9 +23,13 @@
 //    stops, instead of silently overwriting.
 //  * A claim is (start, 64-bit pending mask) held in lane b of the wave for
 //    bucket b, so a wave keeps claims on all buckets in two registers.
-//  * A reader takes one snapshot of every bucket's counters (lane b loads
+//  * A reader takes one snapshot of the buckets' counters (lane b loads
 //    bucket b's tail/head/taken, all lanes the epoch) in a single round trip,
-//    then decides locally; no per-bucket sequence of loads.
+//    then decides locally; no per-bucket sequence of loads.  Queues of more
+//    than kNarrowQueue buckets read only the window's (see l2_pop).
+//  * A writer groups its lanes by bucket and reserves every group's slots in
+//    one round trip (the reference's __match_any_sync grouping, done with
+//    ballots, and one fetch_add per bucket leader, all issued together).
 //  * Counters live on separate 128-byte lines; all are agent-scope atomics.
 #pragma once
 
