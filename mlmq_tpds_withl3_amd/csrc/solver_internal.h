@@ -109,7 +109,6 @@ struct ServeLaunch {
     int waves;                // waves per workgroup: 4, 8 or 16
 };
 
-hipError_t serve_occupancy(int waves, int *blocks_per_cu);
 hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t s);
 hipError_t launch_serve_verify(const ServeLaunch &S, uint32_t n, hipStream_t s);
 hipError_t batch_occupancy(int waves, int *blocks_per_cu);

@@ -1087,14 +1087,6 @@ hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t serve_occupancy(int waves, int *blocks_per_cu) {
-    const void *fn = waves == 16  ? reinterpret_cast<const void *>(&sssp_serve_kernel<16>)
-                     : waves == 8 ? reinterpret_cast<const void *>(&sssp_serve_kernel<8>)
-                     : waves == 4 ? reinterpret_cast<const void *>(&sssp_serve_kernel<4>)
-                                  : reinterpret_cast<const void *>(&sssp_serve_kernel<1>);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
-}
-
 hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t s) {
     if (S.nreq == 0) return hipSuccess;
     SolveArgs a = make_args(L);
