@@ -58,13 +58,4 @@ __device__ __forceinline__ HopBlock empty_block() {
     return b;
 }
 
-// Edge k (< kInlineEdges) of the block stored as plane-major entry `o` of an
-// LDS block array with `cap` entries per plane.
-__device__ __forceinline__ uint2 block_edge_lds(const uint4 *planes, uint32_t cap, uint32_t o,
-                                                uint32_t k) {
-    const uint32_t word = 2 + 2 * k;
-    const uint32_t *p = reinterpret_cast<const uint32_t *>(planes + (word >> 2) * cap + o);
-    return *reinterpret_cast<const uint2 *>(p + (word & 3));
-}
-
 }  // namespace mlmq
