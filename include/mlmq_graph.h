@@ -11,6 +11,10 @@
  *                   graphs); they build the seeded synthetic workloads:
  *                   planar k-nearest-neighbour "road-like" graphs (optionally
  *                   renumbered in Morton order) and symmetrised RMAT graphs.
+ *   reordering   <- graph_transform/reorder_gr.cu (BFS renumbering, optional
+ *                   level split, cross-edge count); the reorder_gr command
+ *                   built beside the library keeps that tool's command line,
+ *                   output files and report lines.
  */
 #ifndef MLMQ_GRAPH_H
 #define MLMQ_GRAPH_H
@@ -50,6 +54,24 @@ double mlmq_graph_weight_scale(const mlmq_host_graph *g);
 int32_t mlmq_graph_centroid_vertex(const mlmq_host_graph *g);
 /* Vertex of largest out-degree (lowest id among ties). */
 int32_t mlmq_graph_max_degree_vertex(const mlmq_host_graph *g);
+/* Vertex renumbering in breadth-first discovery order; replaces the
+ * reorder step of graph_transform/reorder_gr.cu:94-185.  The search starts
+ * at `root` (new id 0) and scans each row in CSR order; vertices it does not
+ * reach follow in ascending old id (:128-134).  layer_split != 0 then cuts
+ * every BFS level (and finally the unreached group), in discovery order,
+ * between the lower half [0, n/2) and the upper half [n/2, n), so that the
+ * lower half holds min(n/2, ceil(end/2)) of the first `end` positions
+ * (:142-156).  perm[new] = old and invperm[old] = new (n entries each,
+ * caller-owned, either may be NULL).  *out gets the relabelled graph: rows in
+ * new order, each row's edges in their old order with renamed destinations
+ * and unchanged weights (:158-177); k-NN coordinates follow their vertices.
+ * Returns 0, -1 for a bad argument (root outside [0, n)), -4 out of memory. */
+int mlmq_graph_reorder_bfs(const mlmq_host_graph *g, int32_t root, int32_t layer_split,
+                           int32_t *perm, int32_t *invperm, mlmq_host_graph **out);
+/* Directed edges (u, v) with u and v on different sides of `cut` (ids below
+ * cut vs the rest): the partition-quality figure graph_transform/
+ * reorder_gr.cu:47-57 reports at cut = n/2.  -1 for a bad argument. */
+int64_t mlmq_graph_cross_edges(const mlmq_host_graph *g, int32_t cut);
 void mlmq_graph_free(mlmq_host_graph *g);
 
 #ifdef __cplusplus

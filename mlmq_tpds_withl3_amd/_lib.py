@@ -33,7 +33,7 @@ GRAPH_SYMBOLS = [
     "mlmq_graph_generate_knn", "mlmq_graph_generate_rmat", "mlmq_graph_read_gr",
     "mlmq_graph_from_csr", "mlmq_graph_write_gr", "mlmq_graph_dims", "mlmq_graph_arrays",
     "mlmq_graph_coords", "mlmq_graph_weight_scale", "mlmq_graph_centroid_vertex", "mlmq_graph_max_degree_vertex",
-    "mlmq_graph_free",
+    "mlmq_graph_reorder_bfs", "mlmq_graph_cross_edges", "mlmq_graph_free",
 ]
 SERVE_SYMBOLS = [
     "mlmq_server_config_default", "mlmq_serve_graph_params_default", "mlmq_server_create",
@@ -231,6 +231,8 @@ def load() -> ctypes.CDLL:
         "mlmq_graph_weight_scale": (ctypes.c_double, [P]),
         "mlmq_graph_centroid_vertex": (i32, [P]),
         "mlmq_graph_max_degree_vertex": (i32, [P]),
+        "mlmq_graph_reorder_bfs": (ctypes.c_int, [P, i32, i32, P, P, ctypes.POINTER(P)]),
+        "mlmq_graph_cross_edges": (i64, [P, i32]),
         "mlmq_graph_free": (None, [P]),
         "mlmq_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
         "mlmq_serve_graph_params_default": (None, [ctypes.POINTER(ServeGraphParams)]),

@@ -504,6 +504,111 @@ int32_t mlmq_graph_max_degree_vertex(const mlmq_host_graph *g) {
     return best;
 }
 
+int mlmq_graph_reorder_bfs(const mlmq_host_graph *g, int32_t root, int32_t layer_split,
+                           int32_t *perm_out, int32_t *invperm_out, mlmq_host_graph **out) {
+    if (!g || !out || g->n <= 0 || root < 0 || root >= g->n) return kInvalid;
+    *out = nullptr;
+    const int32_t n = g->n;
+    mlmq_host_graph *r = nullptr;
+    try {
+        // perm is the BFS queue itself: perm[new] = old in discovery order, and
+        // invperm (old -> new, -1 = not yet seen) is the visited set.
+        std::vector<int32_t> perm((size_t)n), inv((size_t)n, -1);
+        // level_end[i] = one past the last position of BFS level i in perm
+        std::vector<int32_t> level_end;
+        int32_t tail = 0;
+        perm[tail++] = root;
+        inv[root] = 0;
+        int32_t level_stop = 1;
+        for (int32_t head = 0; head < tail; ++head) {
+            if (head == level_stop) {
+                level_end.push_back(level_stop);
+                level_stop = tail;
+            }
+            const int32_t u = perm[head];
+            for (int64_t e = g->row[u]; e < g->row[u + 1]; ++e) {
+                const int32_t v = g->col[e];
+                if (inv[v] < 0) {
+                    inv[v] = tail;
+                    perm[tail++] = v;
+                }
+            }
+        }
+        level_end.push_back(level_stop);
+        // vertices the search did not reach follow in ascending old id
+        for (int32_t v = 0; v < n; ++v)
+            if (inv[v] < 0) {
+                inv[v] = tail;
+                perm[tail++] = v;
+            }
+        if (tail != n) return kInvalid;
+        if (tail > level_stop) level_end.push_back(tail);   // the unreached form one group
+
+        if (layer_split) {
+            // Each group (a BFS level, then the unreached) is cut in discovery
+            // order: its first vertices go to the lower half [0, n/2) so that
+            // the lower half holds min(n/2, ceil(end/2)) of the first `end`
+            // positions, the rest go to the upper half [n/2, n).
+            const int32_t cut = n / 2;
+            std::vector<int32_t> split((size_t)n);
+            int32_t lo = 0, hi = cut, begin = 0;
+            for (const int32_t end : level_end) {
+                const int32_t want = std::min<int32_t>(cut, (int32_t)(((int64_t)end + 1) / 2));
+                const int32_t take = want - lo;
+                for (int32_t i = begin; i < end; ++i)
+                    split[(i - begin < take) ? lo++ : hi++] = perm[i];
+                begin = end;
+            }
+            if (lo != cut || hi != n) return kInvalid;
+            perm.swap(split);
+            for (int32_t i = 0; i < n; ++i) inv[perm[i]] = i;
+        }
+
+        r = new mlmq_host_graph();
+        r->n = n;
+        r->m = g->m;
+        r->weight_scale = g->weight_scale;
+        r->row.assign((size_t)n + 1, 0);
+        for (int32_t i = 0; i < n; ++i)
+            r->row[i + 1] = r->row[i] + (g->row[perm[i] + 1] - g->row[perm[i]]);
+        r->col.resize((size_t)g->m);
+        r->w.resize((size_t)g->m);
+        parallel_for(n, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i) {
+                const int32_t old = perm[i];
+                int64_t o = r->row[i];
+                for (int64_t e = g->row[old]; e < g->row[old + 1]; ++e, ++o) {
+                    r->col[o] = inv[g->col[e]];
+                    r->w[o] = g->w[e];
+                }
+            }
+        });
+        if (!g->xy.empty()) {
+            r->xy.resize(g->xy.size());
+            for (int32_t i = 0; i < n; ++i) {
+                r->xy[2 * (size_t)i] = g->xy[2 * (size_t)perm[i]];
+                r->xy[2 * (size_t)i + 1] = g->xy[2 * (size_t)perm[i] + 1];
+            }
+        }
+        if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+        if (invperm_out) std::memcpy(invperm_out, inv.data(), sizeof(int32_t) * (size_t)n);
+    } catch (const std::bad_alloc &) {
+        delete r;
+        return kNoMem;
+    }
+    *out = r;
+    return kOk;
+}
+
+int64_t mlmq_graph_cross_edges(const mlmq_host_graph *g, int32_t cut) {
+    if (!g || cut < 0 || cut > g->n) return -1;
+    int64_t c = 0;
+    for (int32_t u = 0; u < g->n; ++u)
+        for (int64_t e = g->row[u]; e < g->row[u + 1]; ++e)
+            c += (u < cut) != (g->col[e] < cut);
+    return c;
+}
+
 void mlmq_graph_free(mlmq_host_graph *g) { delete g; }
 
 }  // extern "C"

@@ -10,6 +10,10 @@ arrays, or generated:
   ``weight_scale < 0`` sets the longest edge's weight to ``-weight_scale``.
 * ``rmat(scale, edge_factor, a, b, c, seed, wmin, wmax)`` -- symmetrised RMAT.
 
+``reorder_bfs(root, layer_split)`` renumbers vertices in breadth-first order
+(graph_transform/reorder_gr.cu); the ``reorder_gr`` command beside the
+library is the same step with the reference tool's command line.
+
 The named workloads of the benchmark are in ``WORKLOADS``.
 """
 from __future__ import annotations
@@ -103,6 +107,25 @@ class HostGraph:
 
     def max_degree_vertex(self) -> int:
         return int(self._L.mlmq_graph_max_degree_vertex(self._h))
+
+    def reorder_bfs(self, root: int = 0, layer_split: bool = False):
+        """BFS renumbering from ``root``: returns (graph, perm, invperm) with
+        perm[new] = old and invperm[old] = new (int32[n] each)."""
+        perm = np.empty(self.n, dtype=np.int32)
+        inv = np.empty(self.n, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(self._L.mlmq_graph_reorder_bfs(self._h, int(root), int(bool(layer_split)),
+                                             perm.ctypes.data, inv.ctypes.data, ctypes.byref(h)))
+        g = HostGraph(h, self.kind, dict(self.meta, reorder="bfs_layer_split" if layer_split
+                                         else "bfs", reorder_root=int(root)))
+        return g, perm, inv
+
+    def cross_edges(self, cut: int | None = None) -> int:
+        """Directed edges crossing vertex id ``cut`` (default n // 2)."""
+        c = int(self._L.mlmq_graph_cross_edges(self._h, self.n // 2 if cut is None else int(cut)))
+        if c < 0:
+            raise ValueError("cut outside [0, n]")
+        return c
 
     def close(self) -> None:
         if self._h:

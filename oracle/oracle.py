@@ -10,6 +10,12 @@ scripts/multigpu/supplement_graph.cpp of the reference).
 v1 layout read by SSSP/csr_graph.cu:185-296 and written by
 SSSP/csr_graph.cu:302-418 (header u64 x4 {1, 4, n, m}; n u64 row *end*
 offsets; m u32 destinations; 4 pad bytes when m is odd; m int32 weights).
+
+``bfs_reorder``/``relabel``/``cross_edges`` restate the vertex renumbering of
+graph_transform/reorder_gr.cu (pure Python loops: small graphs only).  The
+reference tool needs nvcc and boost to build, so it is not compiled here;
+these functions are pinned by hand-worked orders and by the invariants of a
+breadth-first order in tests/test_reorder.py.
 """
 from __future__ import annotations
 
@@ -142,3 +148,90 @@ def ref_dijkstra_gr(gr_path: str, src: int) -> np.ndarray:
         subprocess.run([REF_EXPORT, gr_path, str(src), str(n // 2), out_gr, out_i32],
                        check=True, stdout=subprocess.DEVNULL)
         return np.fromfile(out_i32, dtype=np.int32)
+
+
+def bfs_reorder(row, col, root: int, layer_split: bool = False):
+    """Return (perm int32[n], invperm int32[n]) with perm[new] = old.
+
+    Breadth-first discovery order from ``root``, each row scanned in CSR order
+    (graph_transform/reorder_gr.cu:94-126); vertices not reached follow in
+    ascending old id (:128-134).  With ``layer_split``, vertices are grouped by
+    BFS level (unreached: one last group) and each group, in discovery order,
+    sends its first ``min(n//2, (end+1)//2) - placed_low`` vertices to the low
+    half and the rest to the high half (:139-156).
+    """
+    row = np.asarray(row, dtype=np.int64)
+    col = np.asarray(col, dtype=np.int64)
+    n = len(row) - 1
+    if not 0 <= root < n:
+        raise ValueError("root out of range")
+    perm = [0] * n
+    inv = [-1] * n
+    level = [-1] * n
+    queue = [root]
+    inv[root] = 0
+    perm[0] = root
+    level[root] = 0
+    nxt, head = 1, 0
+    while head < len(queue):
+        u = queue[head]
+        head += 1
+        for e in range(int(row[u]), int(row[u + 1])):
+            v = int(col[e])
+            if inv[v] == -1:
+                inv[v] = nxt
+                level[v] = level[u] + 1
+                perm[nxt] = v
+                nxt += 1
+                queue.append(v)
+    for v in range(n):
+        if inv[v] == -1:
+            inv[v] = nxt
+            perm[nxt] = v
+            nxt += 1
+    assert nxt == n
+    if layer_split:
+        cut = n // 2
+        out = [0] * n
+        left, right, begin = 0, cut, 0
+        while begin < n:
+            end = begin + 1
+            while end < n and level[perm[end]] == level[perm[begin]]:
+                end += 1
+            take = min(cut, (end + 1) // 2) - left
+            for i in range(begin, end):
+                if i - begin < take:
+                    out[left] = perm[i]
+                    left += 1
+                else:
+                    out[right] = perm[i]
+                    right += 1
+            begin = end
+        assert left == cut and right == n
+        perm = out
+        for i, v in enumerate(perm):
+            inv[v] = i
+    return np.array(perm, dtype=np.int32), np.array(inv, dtype=np.int32)
+
+
+def relabel(row, col, w, perm, invperm):
+    """New CSR with row i = old row perm[i], edges in old order, destinations
+    renamed through invperm, weights kept (graph_transform/reorder_gr.cu:158-177)."""
+    row = np.asarray(row, dtype=np.int64)
+    perm = np.asarray(perm, dtype=np.int64)
+    deg = (row[1:] - row[:-1])[perm]
+    nrow = np.zeros(len(row), dtype=np.int64)
+    np.cumsum(deg, out=nrow[1:])
+    idx = np.concatenate([np.arange(row[p], row[p + 1]) for p in perm]) if len(col) else \
+        np.zeros(0, dtype=np.int64)
+    ncol = np.asarray(invperm, dtype=np.int32)[np.asarray(col)[idx]]
+    nw = np.asarray(w, dtype=np.int32)[idx]
+    return nrow, ncol.astype(np.int32), nw
+
+
+def cross_edges(row, col, cut: int) -> int:
+    """Directed edges whose ends lie on different sides of ``cut``
+    (graph_transform/reorder_gr.cu:47-57)."""
+    row = np.asarray(row, dtype=np.int64)
+    src = np.repeat(np.arange(len(row) - 1), np.diff(row))
+    return int(np.count_nonzero((src < cut) != (np.asarray(col) < cut)))

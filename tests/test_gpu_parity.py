@@ -191,6 +191,20 @@ def test_rmat_small(solver):
         solve_check(solver, row, col, w, src, **cfg)
 
 
+@pytest.mark.parametrize("split", [False, True])
+def test_bfs_renumbered_graph(solver, split):
+    say(f"BFS-renumbered road-like graph, layer split {split}")
+    g = HostGraph.knn(200_000, 3, seed=7, weight_scale=1e5)
+    row, col, w = g.arrays()
+    ref, _, _ = O.dijkstra(row, col, w, 0)
+    for root in (0, 123_457):
+        r, perm, inv = g.reorder_bfs(root, split)
+        nrow, ncol, nw = r.arrays()
+        src = int(inv[0])
+        solve_check(solver, nrow, ncol, nw, src)
+        assert np.array_equal(solver.dist(), ref[perm])
+
+
 def test_road6m_full(solver):
     say("road6m generate")
     g, src, spec = make_workload("road6m")
