@@ -90,7 +90,12 @@ def main():
                     help="1: touch relaxed children's hop blocks in the atomics' round trip")
     ap.add_argument("--stale-check", type=int, default=2,
                     help="2 coherent, 1 through the L2, 3 late (in the atomics' round trip), 0 none")
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="CPU baseline: at least this many oracle runs (within --cpu-cap)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: keep running the oracle for at least this long")
+    ap.add_argument("--cpu-cap", type=float, default=30.0,
+                    help="CPU baseline: stop starting new oracle runs after this many seconds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="per-launch HBM bytes measured by rocprofv3 --pmc (optional)")
@@ -174,21 +179,40 @@ def main():
     wall = time.perf_counter() - wall0
     dist = solver.dist()
     log(f"timed {args.steps} steps in {wall * 1e3:.1f} ms")
+    # The reference's query wall time: reset + solve + copy of the distances
+    # to (pageable) host memory, per query; outside the timed region.
+    query_wall = []
+    for _ in range(5):
+        q0 = time.perf_counter()
+        solver.solve(src, params)
+        solver.dist()
+        query_wall.append((time.perf_counter() - q0) * 1e3)
 
     # ---- CPU baseline leg: the oracle, also the correctness check ----
+    # Single-threaded, pinned to one core of this process's CPU set.
     from oracle import oracle as O
-    c0 = time.perf_counter()
-    ref, e_r, v_r = O.dijkstra(row, col, w, src)
-    first_cpu = time.perf_counter() - c0
-    correct = bool(np.array_equal(dist, ref))
-    cpu_times = [first_cpu]
-    if not args.no_cpu_baseline:
-        for _ in range(max(0, args.cpu_runs - 1)):
+    cores = sorted(os.sched_getaffinity(0))
+    pinned = cores[0]
+    os.sched_setaffinity(0, {pinned})
+    try:
+        t_start = time.perf_counter()
+        c0 = time.perf_counter()
+        ref, e_r, v_r = O.dijkstra(row, col, w, src)
+        cpu_times = [time.perf_counter() - c0]
+        correct = bool(np.array_equal(dist, ref))
+        while not args.no_cpu_baseline:
+            spent = time.perf_counter() - t_start
+            if spent >= args.cpu_cap or (len(cpu_times) >= args.cpu_runs
+                                         and spent >= args.cpu_seconds):
+                break
             c0 = time.perf_counter()
             O.dijkstra(row, col, w, src)
             cpu_times.append(time.perf_counter() - c0)
+    finally:
+        os.sched_setaffinity(0, set(cores))
     cpu_s = statistics.median(cpu_times)
-    log(f"cpu oracle {cpu_s * 1e3:.1f} ms (runs {len(cpu_times)}), correct={correct}")
+    log(f"cpu oracle {cpu_s * 1e3:.1f} ms (runs {len(cpu_times)}, core {pinned}), "
+        f"correct={correct}")
 
     solve_ms = [t["solve_ms"] for t in timings]
     dev_ms = [t["device_ms"] for t in timings]
@@ -239,10 +263,15 @@ def main():
                      "min": round(min(solve_ms), 4), "max": round(max(solve_ms), 4)},
         "device_ms": {"median": round(statistics.median(dev_ms), 4),
                       "mean": round(statistics.mean(dev_ms), 4)},
+        "query_wall_ms": {"median": round(statistics.median(query_wall), 4),
+                          "runs": len(query_wall)},
         "work": {"relaxations": last["relaxations"], "atomics": last["atomics"],
                  "improvements": last["improvements"], "l2_pushes": last["l2_pushes"],
                  "l2_pops": last["l2_pops"], "stale": last["stale"],
                  "batches": last["batches"], "donated": last["donated"],
+                 "idle_polls": last["idle_polls"],
+                 "idle_fraction": round(last["idle_polls"]
+                                        / max(1, last["idle_polls"] + last["batches"]), 4),
                  "inflation": round(last["relaxations"] / max(1, e_r), 4)},
         "correct": correct,
         "roofline": {
@@ -261,11 +290,14 @@ def main():
             "cores": 1,
             "kind": "port",
             "sample": (f"full {args.workload} query from source {src} by the CPU oracle "
-                       f"(binary-heap Dijkstra, 1 thread), median of {len(cpu_times)} runs: "
-                       f"{cpu_s * 1e3:.1f} ms"),
+                       f"(binary-heap Dijkstra, 1 thread pinned to core {pinned}), median of "
+                       f"{len(cpu_times)} runs ({sum(cpu_times):.1f} s): {cpu_s * 1e3:.1f} ms"),
             "ms": round(cpu_s * 1e3, 3),
+            "runs": len(cpu_times),
             "cpu": cpu_model(),
+            "pinned_core": pinned,
             "nproc": os.cpu_count(),
+            "affinity_cpus": len(cores),
         },
     }
     print(json.dumps(out), flush=True)
