@@ -343,11 +343,16 @@ def run_batch(args):
         times.append(solver.solve_batch(sources, params))
     wall = time.perf_counter() - wall0
     log(f"batch: {args.steps} x {args.queries} queries in {wall * 1e3:.1f} ms")
-    # verify every query against the oracle (thread pool; ctypes releases the GIL)
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
+    # verify every query against the oracle (thread pool; ctypes releases the
+    # GIL); the same pass, timed, is the CPU baseline: P independent Dijkstras
+    pool = min(16, os.cpu_count() or 4)
+    c0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=pool) as ex:
         refs = list(ex.map(lambda s: O.dijkstra(row, col, w, int(s)), sources))
+    cpu_wall = time.perf_counter() - c0
     correct = all(np.array_equal(solver.batch_dist(i), refs[i][0]) for i in range(len(sources)))
     e_r = sum(r[1] for r in refs)
+    b_alg = sum(16 * r[2] + 12 * r[1] for r in refs)
     # comparator: the same queries one at a time (plain solves, parameters
     # tuned for single queries the same way as the single-query mode)
     sp = default_params(delta=delta, timeout_s=60.0)
@@ -363,6 +368,8 @@ def run_batch(args):
     qps = args.queries * args.steps / wall
     seq_qps = args.queries / seq_wall
     last = times[-1]
+    dev_avg_s = statistics.mean(t["device_ms"] for t in times) / 1e3
+    achieved = b_alg / dev_avg_s / 1e9
     out = {
         "metric": METRIC, "value": round(qps, 3), "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -380,7 +387,26 @@ def run_batch(args):
         "sequential": {"queries_per_s": round(seq_qps, 3), "wall_ms": round(seq_wall * 1e3, 2),
                        "solve_ms_median": round(statistics.median(seq_ms), 3)},
         "speedup_vs_sequential": round(qps / seq_qps, 3),
+        "device_ms": {"mean": round(dev_avg_s * 1e3, 4)},
+        "work": {"relaxations": last["relaxations"], "improvements": last["improvements"],
+                 "l2_pushes": last["l2_pushes"], "stale": last["stale"],
+                 "batches": last["batches"], "idle_polls": last["idle_polls"],
+                 "reachable_edges": e_r,
+                 "inflation": round(last["relaxations"] / max(1, e_r), 4)},
         "correct": correct,
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 8), "traffic": None,
+            "kernel": "sssp_batch_kernel", "bytes_per_launch": b_alg,
+        },
+        "cpu_baseline": {
+            "value": round(args.queries / cpu_wall, 3), "unit": "queries/s", "cores": pool,
+            "kind": "port",
+            "sample": (f"the same {args.queries} queries by the CPU oracle (binary-heap "
+                       f"Dijkstra), {pool} independent queries at a time on a thread pool: "
+                       f"{cpu_wall * 1e3:.1f} ms"),
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+        },
     }
     print(json.dumps(out), flush=True)
     solver.close()

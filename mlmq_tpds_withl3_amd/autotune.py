@@ -136,12 +136,17 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
     trail = [(dict(cur), best_t)]
     # Shape first, then the child-block touches and the stale-check mode
     # (with the whole device busy the touches can cost more memory-side
-    # bandwidth than they save), then the queue sizes once more.
-    space = [("delta", _delta_grid(cur["delta"])), ("batch_waves", [4, 8, 16]),
+    # bandwidth than they save), then delta, the bucket count and the queue
+    # sizes once more at the point reached: their best values move with the
+    # shape.
+    space = [("batch_waves", [4, 8, 16]), ("delta", None),
              ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32]),
              ("prefetch", [0, 1]), ("stale_check", [1, 3, 2]),
+             ("delta", None), ("buckets", [8, 16, 32]),
              ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
     for key, values in space:
+        if values is None:           # delta: a grid around the current value
+            values = _delta_grid(cur["delta"])
         for v in values:
             if cur.get(key) == v:
                 continue

@@ -85,6 +85,19 @@ def test_tune_batch_descends_every_coordinate():
     assert r["batch_s"] < 0.03
 
 
+def test_tune_batch_revisits_delta_after_the_shape_moves():
+    # the best delta depends on the touches: only a delta pass after the
+    # touches coordinate finds 400000
+    def cost(d):
+        best = 400000 if d["prefetch"] == 0 else 100000
+        c = 2.0 + 0.5 * abs(math.log2(d["delta"] / best))
+        return c + (0.0 if d["prefetch"] == 0 else 1.5)
+
+    s = FakeSolver(cost)
+    r = A.tune_batch(s, np.arange(4), base=dict(delta=100000, batch_waves=16, prefetch=1))
+    assert r["params"]["prefetch"] == 0 and r["params"]["delta"] == 400000
+
+
 class FakeServer:
     def __init__(self, qps):
         self.qps = qps
