@@ -437,15 +437,18 @@ def run_serve(args):
                     spread_waves=args.spread_waves,
                     max_batch=args.max_batch, max_pending=args.max_pending, verify=1,
                     timeout_s=60.0)
-    graphs, pools, seq_ms, tuned = [], [], [], []
-    ex = ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4))
+    graphs, pools, seq_ms, tuned, cpu_ms = [], [], [], [], []
+    P = min(16, os.cpu_count() or 4)
+    ex = ThreadPoolExecutor(max_workers=P)
     for gi, name in enumerate(names):
         t0 = time.time()
         g, _, spec = make_workload(name)
         row, col, w = g.arrays()
         pool = [int(x) for x in rng.integers(0, g.n, size=args.pool)]
         pools.append(pool)
+        c0 = time.perf_counter()
         refs = list(ex.map(lambda s: O.dijkstra(row, col, w, s)[0], pool))
+        cpu_ms.append((time.perf_counter() - c0) * 1e3 / len(pool))   # per query, P at a time
         gid = server.add_graph(g, delta=spec["delta"])
         for s_, r_ in zip(pool, refs):
             server.set_expected(gid, s_, r_)
@@ -479,7 +482,9 @@ def run_serve(args):
         g.close()
     wsum = sum(mix)
     seq_qps = 1e3 / sum(m / wsum * t for m, t in zip(mix, seq_ms))
-    log(f"one-at-a-time capacity of the mix: {seq_qps:.1f} q/s")
+    cpu_qps = 1e3 / sum(m / wsum * t for m, t in zip(mix, cpu_ms))
+    log(f"one-at-a-time capacity of the mix: {seq_qps:.1f} q/s; CPU oracle, {P} threads: "
+        f"{cpu_qps:.1f} q/s")
     sweep = []
     all_ok = True
     for k, rate in enumerate(rates):
@@ -524,6 +529,14 @@ def run_serve(args):
         "one_at_a_time_qps": round(seq_qps, 2),
         "speedup_vs_one_at_a_time": round(value / seq_qps, 3) if seq_qps else None,
         "sweep": sweep, "correct": all_ok,
+        "cpu_baseline": {
+            "value": round(cpu_qps, 3), "unit": "queries/s", "cores": P, "kind": "port",
+            "sample": (f"the CPU oracle (binary-heap Dijkstra) computing the {args.pool} "
+                       f"verification rows of each graph, {P} independent queries at a time; "
+                       f"mix-weighted capacity from the per-graph times "
+                       + ", ".join(f"{n} {t:.1f} ms" for n, t in zip(names, cpu_ms))),
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+        },
     }
     print(json.dumps(out), flush=True)
     server.close()
