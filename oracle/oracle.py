@@ -136,6 +136,16 @@ def ref_available() -> bool:
     return os.access(REF_EXPORT, os.X_OK)
 
 
+def ref_export_gr(gr_path: str, src: int, cut: int, out_gr: str) -> None:
+    """Have the reference's own export tool write its shortcut-augmented
+    graph (G+) of a .gr file (scripts/multigpu/supplement_graph.cpp:44-63)."""
+    if not ref_available():
+        raise RuntimeError("reference oracle binary not built")
+    with tempfile.TemporaryDirectory() as td:
+        subprocess.run([REF_EXPORT, gr_path, str(src), str(cut), out_gr,
+                        os.path.join(td, "d.i32")], check=True, stdout=subprocess.DEVNULL)
+
+
 def ref_dijkstra_gr(gr_path: str, src: int) -> np.ndarray:
     """Run the reference's own int64 oracle on a .gr file; return int32[n]."""
     if not ref_available():

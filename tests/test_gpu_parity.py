@@ -56,6 +56,16 @@ def test_fixture_golden_and_source_switch(solver):
         assert np.array_equal(solver.dist(), exp), f"source {src}"
 
 
+def test_shortcut_fixture_golden(solver):
+    say("fixture with chain shortcuts (G+): golden distances")
+    solver.load_gr(os.path.join(GOLDEN, "fixture_shortcut.gr"))
+    for src in (0, 1030, 2050):
+        for queue in ("L1SLF_L2DQ", "L1V_L2DQ"):
+            solver.solve(src, timeout_s=TIMEOUT, queue=queue)
+            exp = np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), dtype=np.int32)
+            assert np.array_equal(solver.dist(), exp), (src, queue)
+
+
 @pytest.mark.parametrize("queue", ["L1SLF_L2DQ", "L1N_L2DQ", "L1V_L2DQ"])
 @pytest.mark.parametrize("seed", range(6))
 def test_random_graphs(solver, seed, queue):

@@ -28,6 +28,18 @@ def test_product_reader_matches_numpy_reader():
     assert np.array_equal(row, r2) and np.array_equal(col, c2) and np.array_equal(w, w2)
 
 
+def test_product_reader_on_shortcut_fixture(tmp_path):
+    path = os.path.join(GOLDEN, "fixture_shortcut.gr")
+    g = HostGraph.read_gr(path)
+    row, col, w = g.arrays()
+    r2, c2, w2 = O.read_gr(path)
+    assert g.n == 2052 and g.m == 5644
+    assert np.array_equal(row, r2) and np.array_equal(col, c2) and np.array_equal(w, w2)
+    out = str(tmp_path / "copy.gr")
+    g.write_gr(out)
+    assert open(out, "rb").read() == open(path, "rb").read()
+
+
 def test_product_writer_round_trip(tmp_path):
     g = HostGraph.read_gr(os.path.join(GOLDEN, "fixture.gr"))
     out = str(tmp_path / "copy.gr")

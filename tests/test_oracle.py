@@ -10,8 +10,12 @@ Each pin is independent of the oracle's own code:
     scripts/multigpu/supplement_graph.cpp into oracle/_ref/ (skipped where it
     was not built);
   * the closed form total_work = E_r = sum of out-degrees of reachable
-    vertices (SS/SSSP/main.cu:71) and V_r from a BFS.
+    vertices (SS/SSSP/main.cu:71) and V_r from a BFS;
+  * the reference's second known-answer graph: the fixture with its chain
+    shortcuts (G+, 5644 edges, checks/s{0,1030,2050}.gr -- one file, the
+    same for every source), whose distances equal the fixture's.
 """
+import hashlib
 import os
 
 import numpy as np
@@ -141,3 +145,25 @@ def test_reference_oracle_golden():
     path = os.path.join(GOLDEN, "fixture.gr")
     for src in (0, 1030, 2050):
         assert np.array_equal(O.ref_dijkstra_gr(path, src), golden(src))
+
+
+SHORTCUT_SHA = "59a814ec13ef9aaaf5cadc9f7f52bd4cb1899122bf967be04ded6e023acf5439"
+
+
+def test_shortcut_fixture_known_answers():
+    path = os.path.join(GOLDEN, "fixture_shortcut.gr")
+    assert hashlib.sha256(open(path, "rb").read()).hexdigest() == SHORTCUT_SHA
+    row, col, w = O.read_gr(path)
+    assert len(row) - 1 == 2052 and len(col) == 5644
+    for src in (0, 1030, 2050):
+        d, _, _ = O.dijkstra(row, col, w, src)
+        assert np.array_equal(d, golden(src)), src
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference oracle not built here")
+def test_shortcut_fixture_is_the_reference_export(tmp_path):
+    want = open(os.path.join(GOLDEN, "fixture_shortcut.gr"), "rb").read()
+    for src in (0, 1030, 2050):
+        out = str(tmp_path / f"g{src}.gr")
+        O.ref_export_gr(os.path.join(GOLDEN, "fixture.gr"), src, 2052 // 2, out)
+        assert open(out, "rb").read() == want, src
