@@ -304,8 +304,8 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
                               const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
-                              uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &hungry,
-                              WaveStats &st, PhaseClock<kProf> &clk) {
+                              uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &sunk,
+                              uint32_t &hungry, WaveStats &st, PhaseClock<kProf> &clk) {
     const uint32_t lane = lane_id();
     const bool valid = lane < nin;
     uint32_t v = rec_vertex(rec);
@@ -334,9 +334,12 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t cur = kUnreached;
     HopBlock b = empty_block();
     if (valid && !hot) {
+        // The block first: the compiler waits for every outstanding load at
+        // the join of the stale-check modes' branches, and with the block
+        // already issued that wait covers both loads in one round trip.
+        b = load_block(a.blocks, v);
         if (a.stale_check() == 2) cur = ld_relaxed(a.dist + v);
         else if (a.stale_check() == 1) cur = a.dist[v];
-        b = load_block(a.blocks, v);
     }
     if (kHot) {
         // word-wise select of register values (keeps both sources out of memory)
@@ -453,12 +456,17 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // atomic).  Issued after the atomics and without a branch (lanes without
     // an edge, and all lanes when a.prefetch() is off, touch vertex 0's
     // block), so waiting for the atomics' results takes a fixed vmcnt that
-    // leaves these loads in flight.  The loaded words are consumed one batch
-    // later.
+    // leaves these loads in flight.  A register's previous word is folded
+    // into `sunk` just before the register is reloaded, one batch after its
+    // load: vmcnt counts in issue order, so the wait for this batch's block
+    // loads already covered it.  (Copying the words aside at the end of the
+    // batch instead would wait for these loads there, putting their whole
+    // latency between two batches.)
     if (!kHot) {
 #pragma unroll
         for (int k = 0; k < (int)kInlineEdges; ++k) {
             const uint32_t t = (a.prefetch() && ((on >> k) & 1)) ? c[k] : 0u;
+            sunk ^= pf[k];
             pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
         }
     }
@@ -536,11 +544,11 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
     uint32_t cl_start = 0;      // lane b: start of the claim on bucket b
     uint64_t cl_mask = 0;       // lane b: pending slots of that claim
     uint32_t sleep = 0;
-    uint32_t pf[kInlineEdges], pf_prev[kInlineEdges];   // prefetch results, consumed one batch late
+    uint32_t pf[kInlineEdges];  // prefetch results, consumed one batch late
     uint32_t sunk = 0;
     uint32_t base_hint = 0;     // L2 epoch seen by this wave's last pop
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) pf[k] = pf_prev[k] = 0;
+    for (int k = 0; k < (int)kInlineEdges; ++k) pf[k] = 0;
     bool idle_listed = false;   // counted in the control block's hungry-wave count
 
     for (;;) {
@@ -596,18 +604,13 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf, kHot, kExt, kPre>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, hungry,
+        if (!process_batch<kSLF, kProf, kHot, kExt, kPre>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
                                               st, clk))
             break;
         if (kSLF && hungry && l1.size > a.donate_min()) {
             // (uniform: hungry is a scalar load result)
             if (!l1_donate(ring, l1, uniform(l1.size) / 2, q, st)) break;
             clk.lap(st, kClkFlush);
-        }
-#pragma unroll
-        for (int k = 0; k < (int)kInlineEdges; ++k) {
-            sunk ^= pf_prev[k];
-            pf_prev[k] = pf[k];
         }
         if (credit && l1.size == 0) {
             if (lane == 0) fetch_add(q.ctl + kDoneOff, credit);
@@ -623,7 +626,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
     }
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) sunk ^= pf_prev[k];
+    for (int k = 0; k < (int)kInlineEdges; ++k) sunk ^= pf[k];
     st.v[kStatSpare2] += popc(ballot(sunk == 0x5EEDF00Du));
 }
 
