@@ -61,16 +61,35 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None)
 
 
 def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int = 64,
-                 repeats: int = 1, log=None) -> dict:
+                 repeats: int = 1, lone: int = 3, lone_slack: float | None = 0.25,
+                 log=None) -> dict:
     """Tune one resident graph of a ``serve.Server`` for throughput: each
     candidate serves a burst of `burst` requests (all arriving at once, drawn
-    from `sources`) and is scored by completed queries per second."""
+    from `sources`) and is scored by completed queries per second.
+
+    The same parameters also serve requests that arrive alone (at low load a
+    launch holds one request, spread over many waves), whose latency the
+    burst does not see.  So each candidate also serves `lone` requests one at
+    a time, and a candidate whose median lone latency exceeds the starting
+    point's by more than `lone_slack` is not taken (None: throughput only);
+    the starting point (`base`) should be the single-query tuned parameters."""
     rng = np.random.default_rng(7)
     src = np.asarray(sources, dtype=np.int32)[rng.integers(0, len(sources), size=burst)]
     arrival = np.zeros(burst)
     gsel = np.full(burst, gid, dtype=np.int32)
+    lone_src = [int(x) for x in src[:max(1, lone)]]
     cur = dict(base or {})
     cur.setdefault("delta", 200000)
+
+    def lone_ms():
+        lat = []
+        for s_ in lone_src:
+            res, st = server.run(np.zeros(1), np.full(1, gid, dtype=np.int32),
+                                 np.full(1, s_, dtype=np.int32))
+            if st["completed"] != 1 or st["mismatched"]:
+                return float("inf")
+            lat.append(float(res["latency_ms"][0]))
+        return float(np.median(lat))
 
     def score(params):
         server.set_graph_params(gid, **params)
@@ -78,12 +97,13 @@ def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int
         for _ in range(repeats):
             res, st = server.run(arrival, gsel, src)
             if st["completed"] != burst or st["mismatched"]:
-                return 0.0
+                return 0.0, float("inf")
             best = max(best, st["completed_qps"])
-        return best
+        return best, (lone_ms() if lone_slack is not None else 0.0)
 
-    best_s = score(cur)
-    trail = [(dict(cur), best_s)]
+    best_s, lone0 = score(cur)
+    lone_cap = lone0 * (1.0 + lone_slack) if lone_slack is not None else float("inf")
+    trail = [(dict(cur), best_s, lone0)]
     space = [("delta", _delta_grid(cur["delta"])), ("workgroups", [1, 2, 4, 8, 16, 32]),
              ("waves", [1, 4, 8, 16]), ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
     if not cur.get("light_split"):
@@ -101,14 +121,14 @@ def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int
                 cand["bucket_window"] = min(4, v)
             if key == "min_read":
                 cand["l2_batch"] = max(8, v)
-            s = score(cand)
-            trail.append((cand, s))
+            s, lms = score(cand)
+            trail.append((cand, s, lms))
             if log:
-                log(f"tune graph {gid}: {cand} -> {s:.1f} q/s")
-            if s > best_s:
+                log(f"tune graph {gid}: {cand} -> {s:.1f} q/s, alone {lms:.2f} ms")
+            if s > best_s and lms <= lone_cap:
                 best_s, cur = s, cand
     server.set_graph_params(gid, **cur)
-    return {"params": cur, "qps": best_s, "trail": trail}
+    return {"params": cur, "qps": best_s, "lone_ms_start": lone0, "trail": trail}
 
 
 def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:

@@ -99,8 +99,11 @@ def test_tune_batch_revisits_delta_after_the_shape_moves():
 
 
 class FakeServer:
-    def __init__(self, qps):
+    """Burst rate and lone-request latency from functions of the parameters."""
+
+    def __init__(self, qps, lone=lambda p: 10.0):
         self.qps = qps
+        self.lone = lone
         self.params = {}
         self.calls = 0
 
@@ -110,7 +113,8 @@ class FakeServer:
     def run(self, arrival, graph, src):
         self.calls += 1
         n = len(arrival)
-        return None, {"completed": n, "mismatched": 0, "completed_qps": self.qps(self.params)}
+        res = {"latency_ms": np.full(n, self.lone(self.params))}
+        return res, {"completed": n, "mismatched": 0, "completed_qps": self.qps(self.params)}
 
 
 def test_tune_serving_picks_highest_rate_and_applies_it():
@@ -125,3 +129,20 @@ def test_tune_serving_picks_highest_rate_and_applies_it():
     assert r["params"]["delta"] == 50000 and r["params"]["workgroups"] == 16
     assert r["params"]["prefetch"] == -1
     assert srv.params == r["params"]          # the best point is left applied
+
+
+def test_tune_serving_keeps_lone_latency():
+    # the biggest burst rate comes with delta 50000 and l1_pop 16, but
+    # l1_pop 16 doubles the latency of a request served alone
+    def qps(p):
+        return 1000.0 + (300 if p.get("delta") == 50000 else 0) + (200 if p.get("l1_pop") == 16 else 0)
+
+    def lone(p):
+        return 20.0 if p.get("l1_pop") == 16 else 10.0
+
+    srv = FakeServer(qps, lone)
+    r = A.tune_serving(srv, 0, np.arange(10), base=dict(delta=200000, l1_pop=32), repeats=1)
+    assert r["params"]["delta"] == 50000 and r["params"]["l1_pop"] == 32
+    r = A.tune_serving(srv, 0, np.arange(10), base=dict(delta=200000, l1_pop=32), repeats=1,
+                       lone_slack=None)
+    assert r["params"]["l1_pop"] == 16
