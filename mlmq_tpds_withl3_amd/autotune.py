@@ -26,12 +26,15 @@ def default_delta(row, w) -> int:
     return max(1, int(8 * float(w.mean()))) if len(w) else 1
 
 
-def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None) -> dict:
+def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, confirm: int = 7,
+                log=None) -> dict:
     """Grid search for single queries: every delta in `deltas` crossed with
     `grid` (a list of parameter dicts, default: L1 pop size / L2 top-up
     threshold pairs and two grid sizes), then the late and L2-cached stale
     checks at the four best points.  Each point is the best device time of
-    `repeats` solves.  Returns the best point and the whole table."""
+    `repeats` solves; the four best are then timed `confirm` times more and
+    the lowest median wins (a best-of-two is often a lucky draw).  Returns the
+    best point, its median and the whole table."""
     from .solver import default_params
 
     if grid is None:
@@ -56,8 +59,15 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, log=None)
         for sc in (3, 1):
             if bp.get("stale_check", 2) != sc:
                 measure(dict(bp, stale_check=sc))
-    best = min(table, key=lambda t: t[1])
-    return {"params": best[0], "device_ms": best[1], "table": table}
+    finals = []
+    for bp, _ in sorted(table, key=lambda t: t[1])[:4]:
+        p = default_params(timeout_s=30.0, **bp)
+        runs = sorted(solver.solve(src, p)["device_ms"] for _ in range(max(1, confirm)))
+        finals.append((bp, runs[len(runs) // 2]))
+        if log:
+            log(f"tune single, confirm: {bp} -> median {finals[-1][1]:.3f} ms")
+    best = min(finals, key=lambda t: t[1])
+    return {"params": best[0], "device_ms": best[1], "table": table, "confirmed": finals}
 
 
 def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int = 64,

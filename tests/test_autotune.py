@@ -56,8 +56,10 @@ def test_tune_single_finds_best_and_tries_stale_modes():
     at_best = [d for d in s.seen if d["delta"] == 100000 and d["work_blocks"] == 512
                and d["buckets"] == 4 and d["l1_pop"] == 32 and d["min_read"] == 16]
     assert {d["stale_check"] for d in at_best} == {1, 2, 3}
-    # every table entry is one measured point
-    assert len(r["table"]) == len(s.seen)
+    # every table entry is one measured point; the four best were then
+    # timed seven times more each
+    assert len(r["table"]) + 4 * 7 == len(s.seen)
+    assert len(r["confirmed"]) == 4
 
 
 def test_tune_single_custom_grid():
@@ -146,3 +148,16 @@ def test_tune_serving_keeps_lone_latency():
     r = A.tune_serving(srv, 0, np.arange(10), base=dict(delta=200000, l1_pop=32), repeats=1,
                        lone_slack=None)
     assert r["params"]["l1_pop"] == 16
+
+
+def test_tune_single_confirms_against_lucky_draws():
+    # point A is usually 6 ms but once measured 4 ms; B is always 5 ms
+    draws = {"A": iter([4.0] + [6.0] * 100)}
+
+    def cost(d):
+        return next(draws["A"]) if d["l1_pop"] == 64 else 5.0
+
+    s = FakeSolver(cost)
+    grid = [dict(l1_pop=64), dict(l1_pop=32)]
+    r = A.tune_single(s, 0, (100,), grid=grid, repeats=1)
+    assert r["params"]["l1_pop"] == 32 and r["device_ms"] == 5.0
