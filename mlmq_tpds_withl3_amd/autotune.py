@@ -30,7 +30,7 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, confirm: 
                 log=None) -> dict:
     """Grid search for single queries: every delta in `deltas` crossed with
     `grid` (a list of parameter dicts, default: L1 pop size / L2 top-up
-    threshold pairs and two grid sizes), then the late and L2-cached stale
+    threshold pairs, bucket counts and three grid sizes), then the late and L2-cached stale
     checks at the four best points.  Each point is the best device time of
     `repeats` solves; the four best are then timed `confirm` times more and
     the lowest median wins (a best-of-two is often a lucky draw).  Returns the
@@ -40,7 +40,7 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, confirm: 
     if grid is None:
         grid = [dict(l1_pop=pop, min_read=mr, l2_batch=lb, work_blocks=wb, buckets=nb,
                      bucket_window=min(4, nb))
-                for wb in (256, 512) for nb in (16, 4, 2)
+                for wb in (256, 384, 512) for nb in (16, 4, 2)
                 for pop, mr, lb in ((32, 32, 8), (32, 32, 32), (32, 16, 16), (32, 8, 8), (64, 8, 8))]
     table = []
 
