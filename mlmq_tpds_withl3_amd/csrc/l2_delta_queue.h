@@ -200,10 +200,16 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
     // taken record has been credited back (its expansion, and the L1 work it
     // spawned, finished), so no record of the head bucket can still lower a
     // distance in the next one: a delta-stepping phase barrier.
+    // A claim this wave still holds on the head bucket does not hold the
+    // window back: once every reserved record is taken, a claim can only
+    // cover slots past the tail (claims race ahead of reservations), which a
+    // later push fills when the bucket is back in the window; if every wave
+    // held such a claim and waited for it, no wave would advance and the
+    // records in later buckets would never be read.
     const uint32_t b0 = base % q.nbuckets;
     const uint32_t live = rd ? t - k : 0;
     const bool b0_read = read_lane(rd ? 1u : 0u, b0) != 0;
-    if (b0_read && read_lane(live, b0) == 0 && !((claim_mask >> b0) & 1)) {
+    if (b0_read && read_lane(live, b0) == 0) {
         uint32_t total = uniform(wave_sum(live));
         if (wide && total == 0) {
             uint32_t t2 = 0, k2 = 0;
