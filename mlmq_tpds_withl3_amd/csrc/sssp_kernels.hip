@@ -458,10 +458,11 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // block), so waiting for the atomics' results takes a fixed vmcnt that
     // leaves these loads in flight.  A register's previous word is folded
     // into `sunk` just before the register is reloaded, one batch after its
-    // load: vmcnt counts in issue order, so the wait for this batch's block
-    // loads already covered it.  (Copying the words aside at the end of the
-    // batch instead would wait for these loads there, putting their whole
-    // latency between two batches.)
+    // load (vmcnt counts in issue order, so this batch's block-load wait
+    // already covered it).  The compiler still moves these loop-carried words
+    // between registers at the loop's back edge, which waits for the loads
+    // there; the touches pay for themselves regardless (without them the
+    // batch's block loads miss the L2 more often).
     if (!kHot) {
 #pragma unroll
         for (int k = 0; k < (int)kInlineEdges; ++k) {
