@@ -9,7 +9,8 @@
 //      load-balanced wave-wide edge loop and agent-scope atomicMin;
 //   3. push improved (vertex, dist) records through L1, spilling to L2
 //      (core/src/ml_queue.cu:321-355);
-//   4. return the L2 credit once L1 is empty (:565-572).
+//   4. return the L2 credit when the wave runs out of work (the reference
+//      returns it whenever L1 is empty, :565-572).
 // Global wave 0 also detects termination while idle, replacing the separate
 // manager kernel (:623-674).  Queue publication needs no manager at all (see
 // l2_delta_queue.h), so no wave waits on a particular other workgroup: the
@@ -535,12 +536,16 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
     const L2Queue &q = a.q;
     L1State l1{0u, 0u};
     // Records taken from L2 whose work is not finished.  Credits are returned
-    // with a relaxed atomic add: termination needs every tail reservation a
-    // record's expansion made to be performed before its credit, and each
-    // reservation's result (the slot index) was waited for before the
-    // record's children were written, so program order already gives that.
-    // A release would also write back this XCD's L2 and wait for the
-    // in-flight prefetch loads.
+    // only when the wave runs out of work (the idle path below): a wave with
+    // work cannot let the solve end anyway, and returning them whenever L1
+    // drains mid-run put a non-returning atomic in flight that the next
+    // batch's loads waited behind (vmcnt counts in issue order).  They are
+    // returned with a relaxed atomic add: termination needs every tail
+    // reservation a record's expansion made to be performed before its
+    // credit, and each reservation's result (the slot index) was waited for
+    // before the record's children were written, so program order already
+    // gives that.  A release would also write back this XCD's L2 and wait for
+    // the in-flight prefetch loads.
     uint32_t credit = 0;
     uint32_t cl_start = 0;      // lane b: start of the claim on bucket b
     uint64_t cl_mask = 0;       // lane b: pending slots of that claim
@@ -612,11 +617,6 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
             // (uniform: hungry is a scalar load result)
             if (!l1_donate(ring, l1, uniform(l1.size) / 2, q, st)) break;
             clk.lap(st, kClkFlush);
-        }
-        if (credit && l1.size == 0) {
-            if (lane == 0) fetch_add(q.ctl + kDoneOff, credit);
-            credit = 0;
-            clk.lap(st, kClkCredit);
         }
         if ((st.v[kStatBatches] & 31) == 0) {
             if (uniform(ld_relaxed(q.ctl + kExitOff))) break;
