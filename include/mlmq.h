@@ -74,7 +74,8 @@ typedef struct mlmq_params {
     int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
     int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */
     int32_t  l2_batch;        /* records per L2 claim (l2_batch_size), 1..64, default 8 */
-    int32_t  min_read;        /* top up from L2 below this many (MIN_READ_GRA), default 8 */
+    int32_t  min_read;        /* top up from L2 below this many (MIN_READ_GRA), 1..64, default 8
+                                 (0 is refused: no wave would ever take the seed) */
     int64_t  bucket_capacity; /* ring slots per bucket (rounded up to 2^k); 0 = auto */
     double   timeout_s;       /* device watchdog; <= 0 means 60 s */
     int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
@@ -183,6 +184,12 @@ typedef struct mlmq_graph_info {
 } mlmq_graph_info;
 
 void        mlmq_params_default(mlmq_params *p);
+/* Range check of a parameter set, as mlmq_solve (batch = 0) or
+ * mlmq_solve_batch (batch = 1) applies it before touching the device:
+ * MLMQ_OK, MLMQ_E_INVALID or MLMQ_E_UNSUPPORTED.  Needs no device.  No
+ * reference counterpart (the reference fixes its settings at compile time,
+ * core/include/adaptive.h:35-59). */
+int         mlmq_params_check(const mlmq_params *p, int32_t batch);
 int         mlmq_create(int device, mlmq_ctx **out);
 int         mlmq_destroy(mlmq_ctx *ctx);
 const char *mlmq_last_error(const mlmq_ctx *ctx);

@@ -71,7 +71,7 @@ typedef struct mlmq_serve_graph_params {
     int32_t  buckets;         /* 1..64; 0 = 16 */
     int32_t  bucket_window;   /* 1..buckets; 0 = 4 */
     int32_t  l2_batch;        /* 1..64; 0 = 8 */
-    int32_t  min_read;        /* 0..64; -1 = 8 */
+    int32_t  min_read;        /* 1..64; 0 or -1 = 8 */
     int32_t  l1_pop;          /* 1..64; 0 = 32 */
     int32_t  l1_spill;        /* 1..64; 0 = 64 */
     int32_t  workgroups;      /* workgroups per request, 1..64; 0 = auto from the edge count */
@@ -123,6 +123,12 @@ typedef struct mlmq_serve_stats {
 
 void mlmq_server_config_default(mlmq_server_config *cfg);
 void mlmq_serve_graph_params_default(mlmq_serve_graph_params *p);
+
+/* Defaults filled in and ranges checked for a graph of n vertices and m
+ * edges, as mlmq_server_add_* applies them (MLMQ_OK or MLMQ_E_INVALID).
+ * Needs no device. */
+int mlmq_serve_graph_params_resolve(const mlmq_serve_graph_params *in, int32_t n, int64_t m,
+                                    mlmq_serve_graph_params *out);
 
 int mlmq_server_create(int device, const mlmq_server_config *cfg, mlmq_server **out);
 int mlmq_server_destroy(mlmq_server *s);

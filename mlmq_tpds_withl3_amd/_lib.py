@@ -24,7 +24,7 @@ QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1, "L1V_L2DQ": 0}
 
 # Every exported symbol the headers declare (checked by the CPU test suite).
 SOLVER_SYMBOLS = [
-    "mlmq_params_default", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
+    "mlmq_params_default", "mlmq_params_check", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
     "mlmq_load_csr", "mlmq_load_gr", "mlmq_attach_device_csr", "mlmq_graph_stats",
     "mlmq_graph_size", "mlmq_solve", "mlmq_copy_dist", "mlmq_dist_device_ptr",
     "mlmq_device_shape", "mlmq_probe_copy", "mlmq_solve_batch", "mlmq_copy_batch_dist", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
@@ -36,7 +36,8 @@ GRAPH_SYMBOLS = [
     "mlmq_graph_reorder_bfs", "mlmq_graph_cross_edges", "mlmq_graph_free",
 ]
 SERVE_SYMBOLS = [
-    "mlmq_server_config_default", "mlmq_serve_graph_params_default", "mlmq_server_create",
+    "mlmq_server_config_default", "mlmq_serve_graph_params_default",
+    "mlmq_serve_graph_params_resolve", "mlmq_server_create",
     "mlmq_server_destroy", "mlmq_server_last_error", "mlmq_server_add_csr", "mlmq_server_add_gr",
     "mlmq_server_set_graph_params", "mlmq_server_graph_size", "mlmq_server_slot_bytes",
     "mlmq_server_set_expected", "mlmq_server_clear_expected", "mlmq_server_run",
@@ -194,6 +195,7 @@ def load() -> ctypes.CDLL:
     i32, i64, u64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
     sig = {
         "mlmq_params_default": (None, [ctypes.POINTER(Params)]),
+        "mlmq_params_check": (ctypes.c_int, [ctypes.POINTER(Params), i32]),
         "mlmq_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
         "mlmq_destroy": (ctypes.c_int, [P]),
         "mlmq_last_error": (ctypes.c_char_p, [P]),
@@ -236,6 +238,8 @@ def load() -> ctypes.CDLL:
         "mlmq_graph_free": (None, [P]),
         "mlmq_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
         "mlmq_serve_graph_params_default": (None, [ctypes.POINTER(ServeGraphParams)]),
+        "mlmq_serve_graph_params_resolve": (ctypes.c_int, [ctypes.POINTER(ServeGraphParams), i32, i64,
+                                                           ctypes.POINTER(ServeGraphParams)]),
         "mlmq_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerConfig),
                                               ctypes.POINTER(P)]),
         "mlmq_server_destroy": (ctypes.c_int, [P]),

@@ -338,6 +338,42 @@ int ensure_queue(mlmq_ctx *c, uint32_t nbuckets, uint32_t cap_log2) {
     return MLMQ_OK;
 }
 
+// Range checks shared by mlmq_solve and mlmq_solve_batch (and exported as
+// mlmq_params_check).  min_read must be at least 1: a wave tops up from L2
+// only while it holds fewer than min_read records, so with 0 no wave would
+// ever take the seed and the solve could only end at the watchdog.
+int check_params(const mlmq_params &p, bool batch, std::string *why) {
+    auto bad = [&](int code, const char *msg) {
+        if (why) *why = msg;
+        return code;
+    };
+    const bool fifo = p.queue_mode == MLMQ_QUEUE_L1V_L2DQ;
+    const bool slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ || fifo;
+    if (batch && p.queue_mode != MLMQ_QUEUE_L1SLF_L2DQ)
+        return bad(MLMQ_E_UNSUPPORTED, "batched solves use L1SLF_L2DQ");
+    if (!slf && p.queue_mode != MLMQ_QUEUE_L1N_L2DQ) return bad(MLMQ_E_UNSUPPORTED, "queue mode not implemented");
+    if (p.delta == 0 || p.delta > 0x7FFFFFFFu) return bad(MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
+    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return bad(MLMQ_E_INVALID, "buckets must be in [1, 64]");
+    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return bad(MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
+    if (p.l2_batch < 1 || p.l2_batch > 64) return bad(MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
+    if (p.min_read < 1 || p.min_read > 64) return bad(MLMQ_E_INVALID, "min_read must be in [1, 64]");
+    if (p.l1_pop < 1 || p.l1_pop > 64) return bad(MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
+    if (p.l1_spill < 1 || p.l1_spill > 64) return bad(MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
+    if (p.l1_donate < 0 || p.l1_donate > 64) return bad(MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
+    if (batch && p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
+        return bad(MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
+    if (p.stale_check < 0 || p.stale_check > 3) return bad(MLMQ_E_INVALID, "stale_check must be 0, 1, 2 or 3");
+    if (p.l1_blocks != 0 && p.l1_blocks != 1) return bad(MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
+    if (p.idle_backoff < 0 || p.idle_backoff > 64) return bad(MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
+    if (p.light_split && (p.hub_split || p.defer_heavy))
+        return bad(MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
+    if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
+        return bad(MLMQ_E_INVALID, "l1_blocks excludes light_split, defer_heavy and strict_window");
+    if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
+        return bad(MLMQ_E_INVALID, "negative size parameter");
+    return MLMQ_OK;
+}
+
 }  // namespace
 
 int mlmq::ctx_prepare_light(mlmq_ctx *c, uint32_t delta) {
@@ -364,7 +400,12 @@ int mlmq::ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v) {
 
 extern "C" {
 
-const char *mlmq_version(void) { return "mlmq-gfx950 0.1"; }
+const char *mlmq_version(void) { return "mlmq-gfx950 0.2"; }
+
+int mlmq_params_check(const mlmq_params *p, int32_t batch) {
+    if (!p) return MLMQ_E_INVALID;
+    return check_params(*p, batch != 0, nullptr);
+}
 
 void mlmq_params_default(mlmq_params *p) {
     if (!p) return;
@@ -558,31 +599,17 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
     mlmq_params p;
     if (params) p = *params;
     else mlmq_params_default(&p);
+    {
+        std::string why;
+        const int prc = check_params(p, false, &why);
+        if (prc) return fail(c, prc, why);
+    }
     const bool fifo = p.queue_mode == MLMQ_QUEUE_L1V_L2DQ;
     const bool slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ || fifo;
-    if (!slf && p.queue_mode != MLMQ_QUEUE_L1N_L2DQ)
-        return fail(c, MLMQ_E_UNSUPPORTED, "queue mode not implemented");
-    if (p.delta == 0 || p.delta > 0x7FFFFFFFu) return fail(c, MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
-    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return fail(c, MLMQ_E_INVALID, "buckets must be in [1, 64]");
-    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(c, MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
-    if (p.l2_batch < 1 || p.l2_batch > 64) return fail(c, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
-    if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
-    if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
-    if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
-    if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
-    if (p.stale_check < 0 || p.stale_check > 3) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1, 2 or 3");
-    if (p.l1_blocks != 0 && p.l1_blocks != 1) return fail(c, MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
-    if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
-    if (p.light_split && (p.hub_split || p.defer_heavy))
-        return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
-    if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
-        return fail(c, MLMQ_E_INVALID, "l1_blocks excludes light_split, defer_heavy and strict_window");
     if (p.light_split) {
         const int lrc = ensure_light_counts(c, p.delta);
         if (lrc) return lrc;
     }
-    if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)
-        return fail(c, MLMQ_E_INVALID, "negative size parameter");
     HIP_TRY(c, hipSetDevice(c->device));
 
     int occ = 0;
@@ -612,6 +639,9 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
 
     int rc = ensure_dist(c);
     if (rc) return rc;
+    // The reset below overwrites the last solve's distances: until this solve
+    // succeeds there is no result to copy out.
+    c->solved = false;
 
     const double timeout_s = p.timeout_s > 0 ? p.timeout_s : 60.0;
     uint64_t retries = 0;
@@ -734,29 +764,15 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     mlmq_params p;
     if (params) p = *params;
     else mlmq_params_default(&p);
-    if (p.queue_mode != MLMQ_QUEUE_L1SLF_L2DQ)
-        return fail(c, MLMQ_E_UNSUPPORTED, "batched solves use L1SLF_L2DQ");
-    if (p.delta == 0 || p.delta > 0x7FFFFFFFu) return fail(c, MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
-    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return fail(c, MLMQ_E_INVALID, "buckets must be in [1, 64]");
-    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(c, MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
-    if (p.l2_batch < 1 || p.l2_batch > 64) return fail(c, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
-    if (p.min_read < 0 || p.min_read > 64) return fail(c, MLMQ_E_INVALID, "min_read must be in [0, 64]");
-    if (p.l1_pop < 1 || p.l1_pop > 64) return fail(c, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
-    if (p.l1_spill < 1 || p.l1_spill > 64) return fail(c, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
-    if (p.l1_donate < 0 || p.l1_donate > 64) return fail(c, MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
-    if (p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
-        return fail(c, MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
-    if (p.stale_check < 0 || p.stale_check > 3) return fail(c, MLMQ_E_INVALID, "stale_check must be 0, 1, 2 or 3");
-    if (p.idle_backoff < 0 || p.idle_backoff > 64) return fail(c, MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
-    if (p.light_split && (p.hub_split || p.defer_heavy))
-        return fail(c, MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
-    if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
-        return fail(c, MLMQ_E_INVALID, "l1_blocks excludes light_split, defer_heavy and strict_window");
+    {
+        std::string why;
+        const int prc = check_params(p, true, &why);
+        if (prc) return fail(c, prc, why);
+    }
     if (p.light_split) {
         const int lrc = ensure_light_counts(c, p.delta);
         if (lrc) return lrc;
     }
-    if (p.work_blocks < 0 || p.bucket_capacity < 0) return fail(c, MLMQ_E_INVALID, "negative size parameter");
     for (int32_t i = 0; i < nq; ++i)
         if (sources[i] < 0 || sources[i] >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -773,6 +789,10 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     uint32_t cap_log2 = ceil_log2(cap);
     if (cap_log2 > 31) return fail(c, MLMQ_E_INVALID, "bucket capacity too large");
 
+    // The launch overwrites the last batch's rows: until it succeeds there is
+    // no batch result to copy out.
+    c->b_solved = false;
+    c->b_nq = 0;
     // per-query rows and bookkeeping
     const uint64_t words = (uint64_t)nq * (uint64_t)c->n;
     if (c->b_dist_words < words) {

@@ -98,17 +98,24 @@ int hip_fail(mlmq_server *s, hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(srv, e_, #expr); \
     } while (0)
 
-int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n, int64_t m,
-                   mlmq_serve_graph_params *out, uint32_t *cap_log2) {
+// Defaults and range checks of per-graph serving parameters.  min_read must
+// be at least 1 (a wave tops up from L2 only below min_read records, so 0
+// would leave the seed untaken until the watchdog); 0 and -1 select 8.
+int resolve_params_why(const mlmq_serve_graph_params *in, int32_t n, int64_t m,
+                       mlmq_serve_graph_params *out, uint32_t *cap_log2, std::string *why) {
+    auto fail = [&](int code, const char *msg) {
+        if (why) *why = msg;
+        return code;
+    };
     mlmq_serve_graph_params p;
     if (in) p = *in;
     else mlmq_serve_graph_params_default(&p);
     if (p.delta == 0) p.delta = 200000;
-    if (p.delta > 0x7FFFFFFFu) return fail(s, MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
+    if (p.delta > 0x7FFFFFFFu) return fail(MLMQ_E_INVALID, "delta must be in [1, INT_MAX]");
     if (p.buckets == 0) p.buckets = 16;
     if (p.bucket_window == 0) p.bucket_window = std::min(4, p.buckets);
     if (p.l2_batch == 0) p.l2_batch = 8;
-    if (p.min_read < 0) p.min_read = 8;
+    if (p.min_read <= 0) p.min_read = 8;
     if (p.l1_pop == 0) p.l1_pop = 32;
     if (p.l1_spill == 0) p.l1_spill = 64;
     if (p.waves == 0) p.waves = 8;
@@ -116,29 +123,37 @@ int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n,
         const double k = std::round((double)m / 4.0e6);
         p.workgroups = (int32_t)std::min(16.0, std::max(1.0, k));
     }
-    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return fail(s, MLMQ_E_INVALID, "buckets must be in [1, 64]");
-    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(s, MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
-    if (p.l2_batch < 1 || p.l2_batch > 64) return fail(s, MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
-    if (p.min_read > 64) return fail(s, MLMQ_E_INVALID, "min_read must be in [0, 64]");
-    if (p.l1_pop < 1 || p.l1_pop > 64) return fail(s, MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
-    if (p.l1_spill < 1 || p.l1_spill > 64) return fail(s, MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
-    if (p.workgroups < 1 || p.workgroups > 64) return fail(s, MLMQ_E_INVALID, "workgroups must be in [1, 64]");
+    if (p.buckets < 1 || p.buckets > (int)mlmq::kMaxBuckets) return fail(MLMQ_E_INVALID, "buckets must be in [1, 64]");
+    if (p.bucket_window < 1 || p.bucket_window > p.buckets) return fail(MLMQ_E_INVALID, "bucket_window must be in [1, buckets]");
+    if (p.l2_batch < 1 || p.l2_batch > 64) return fail(MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
+    if (p.min_read > 64) return fail(MLMQ_E_INVALID, "min_read must be in [1, 64]");
+    if (p.l1_pop < 1 || p.l1_pop > 64) return fail(MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
+    if (p.l1_spill < 1 || p.l1_spill > 64) return fail(MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
+    if (p.workgroups < 1 || p.workgroups > 64) return fail(MLMQ_E_INVALID, "workgroups must be in [1, 64]");
     if (p.waves != 1 && p.waves != 4 && p.waves != 8 && p.waves != 16)
-        return fail(s, MLMQ_E_INVALID, "waves must be 1, 4, 8 or 16");
-    if (p.bucket_capacity < 0) return fail(s, MLMQ_E_INVALID, "negative bucket_capacity");
-    if (p.hub_split != 0 && p.hub_split != 1) return fail(s, MLMQ_E_INVALID, "hub_split must be 0 or 1");
-    if (p.light_split != 0 && p.light_split != 1) return fail(s, MLMQ_E_INVALID, "light_split must be 0 or 1");
-    if (p.light_split && p.hub_split) return fail(s, MLMQ_E_INVALID, "light_split excludes hub_split");
-    if (p.prefetch < -1 || p.prefetch > 1) return fail(s, MLMQ_E_INVALID, "prefetch must be -1, 0 or 1");
+        return fail(MLMQ_E_INVALID, "waves must be 1, 4, 8 or 16");
+    if (p.bucket_capacity < 0) return fail(MLMQ_E_INVALID, "negative bucket_capacity");
+    if (p.hub_split != 0 && p.hub_split != 1) return fail(MLMQ_E_INVALID, "hub_split must be 0 or 1");
+    if (p.light_split != 0 && p.light_split != 1) return fail(MLMQ_E_INVALID, "light_split must be 0 or 1");
+    if (p.light_split && p.hub_split) return fail(MLMQ_E_INVALID, "light_split excludes hub_split");
+    if (p.prefetch < -1 || p.prefetch > 1) return fail(MLMQ_E_INVALID, "prefetch must be -1, 0 or 1");
     if (p.stale_check < -1 || p.stale_check > 3)
-        return fail(s, MLMQ_E_INVALID, "stale_check must be -1, 0, 1, 2 or 3");
+        return fail(MLMQ_E_INVALID, "stale_check must be -1, 0, 1, 2 or 3");
     uint64_t cap = (uint64_t)p.bucket_capacity;
     if (cap == 0) cap = std::max<uint64_t>((uint64_t)n / 32, 8192);
     const uint32_t c = ceil_log2(cap);
-    if (c > 30) return fail(s, MLMQ_E_INVALID, "bucket capacity too large");
+    if (c > 30) return fail(MLMQ_E_INVALID, "bucket capacity too large");
     *out = p;
     *cap_log2 = c;
     return MLMQ_OK;
+}
+
+int resolve_params(mlmq_server *s, const mlmq_serve_graph_params *in, int32_t n, int64_t m,
+                   mlmq_serve_graph_params *out, uint32_t *cap_log2) {
+    std::string why;
+    const int rc = resolve_params_why(in, n, m, out, cap_log2, &why);
+    if (rc) s->err = why;
+    return rc;
 }
 
 void size_slots(GraphEntry &g) {
@@ -318,6 +333,13 @@ int mlmq_server_destroy(mlmq_server *s) {
 }
 
 const char *mlmq_server_last_error(const mlmq_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int mlmq_serve_graph_params_resolve(const mlmq_serve_graph_params *in, int32_t n, int64_t m,
+                                    mlmq_serve_graph_params *out) {
+    if (!out || n <= 0 || m < 0) return MLMQ_E_INVALID;
+    uint32_t c = 0;
+    return resolve_params_why(in, n, m, out, &c, nullptr);
+}
 
 int mlmq_server_add_csr(mlmq_server *s, int32_t n, int64_t m, const int64_t *row, const int32_t *col,
                         const int32_t *w, const mlmq_serve_graph_params *p, int32_t *graph_id) {

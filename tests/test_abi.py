@@ -111,3 +111,61 @@ def test_ctypes_layouts_match_c(tmp_path):
         assert int(out[cname]) == ctypes.sizeof(cls), cname
         for f in fields or []:
             assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
+
+
+def test_min_read_zero_is_refused_before_any_launch():
+    """min_read = 0 would leave the seed in L2 for ever (a wave tops up only
+    below min_read records) and the solve would spin until the watchdog: the
+    check every solve entry point runs first refuses it."""
+    L = _lib.load()
+    p = _lib.Params()
+    L.mlmq_params_default(ctypes.byref(p))
+    assert L.mlmq_params_check(ctypes.byref(p), 0) == 0
+    assert L.mlmq_params_check(ctypes.byref(p), 1) == 0
+    for bad in (0, -1, 65):
+        p.min_read = bad
+        assert L.mlmq_params_check(ctypes.byref(p), 0) == -1, bad
+        assert L.mlmq_params_check(ctypes.byref(p), 1) == -1, bad
+    p.min_read = 1
+    assert L.mlmq_params_check(ctypes.byref(p), 0) == 0
+    p.min_read = 64
+    assert L.mlmq_params_check(ctypes.byref(p), 0) == 0
+
+
+def test_params_check_ranges():
+    L = _lib.load()
+
+    def check(batch=0, **kw):
+        p = _lib.Params()
+        L.mlmq_params_default(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return L.mlmq_params_check(ctypes.byref(p), batch)
+
+    assert check(delta=0) == -1
+    assert check(buckets=65) == -1 and check(buckets=4, bucket_window=5) == -1
+    assert check(l2_batch=0) == -1 and check(l1_pop=65) == -1
+    assert check(stale_check=4) == -1 and check(work_blocks=-1) == -1
+    assert check(light_split=1, hub_split=1) == -1
+    assert check(queue_mode=5) == -9
+    assert check(queue_mode=0) == 0 and check(queue_mode=0, batch=1) == -9
+    assert check(batch=1, batch_waves=3) == -1 and check(batch=1, batch_waves=16) == 0
+
+
+def test_serve_params_min_read_default_and_range():
+    """Serving parameters: 0 and -1 select the default 8, never 0 itself;
+    above 64 is refused."""
+    L = _lib.load()
+    for given, want in ((0, 8), (-1, 8), (1, 1), (64, 64)):
+        g = _lib.ServeGraphParams()
+        L.mlmq_serve_graph_params_default(ctypes.byref(g))
+        g.min_read = given
+        out = _lib.ServeGraphParams()
+        assert L.mlmq_serve_graph_params_resolve(ctypes.byref(g), 1000, 4000, ctypes.byref(out)) == 0
+        assert out.min_read == want
+    g.min_read = 65
+    assert L.mlmq_serve_graph_params_resolve(ctypes.byref(g), 1000, 4000, ctypes.byref(out)) == -1
+    g = _lib.ServeGraphParams()   # all zero: every default filled in
+    assert L.mlmq_serve_graph_params_resolve(ctypes.byref(g), 1000, 4000, ctypes.byref(out)) == 0
+    assert (out.delta, out.buckets, out.bucket_window, out.l2_batch, out.min_read, out.l1_pop,
+            out.waves, out.workgroups) == (200000, 16, 4, 8, 8, 32, 8, 1)

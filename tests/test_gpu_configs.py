@@ -1,0 +1,174 @@
+"""Parity at the full sizes of the benchmark configurations (needs an MI355X).
+
+Every distance of every query is compared with the CPU oracle:
+
+* road6m (6,262,104-vertex 3-NN graph, Morton order, centroid source) at each
+  delta of its sweep, 50k / 100k / 200k / 400k;
+* RMAT scale 22, edge factor 16, max-degree source, at the tuned point
+  (light/heavy split, delta 50, 64 buckets) and at the reference's defaults,
+  in both L1 modes;
+* 256 seeded sources on the 1,070,376-vertex road graph in one batched
+  launch, every row checked (the reference's per-source check,
+  SSSP/main.cu:195-209);
+* a Poisson serving run over road264k, road6m and RMAT-20 chosen 50/30/20 %,
+  with uniformly random sources: every served row is compared on the device
+  with its oracle row, and every 8th row is also copied out and compared on
+  the host.
+
+Oracle rows are computed on a thread pool (the oracle is C; ctypes releases
+the GIL).
+"""
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from mlmq_tpds_withl3_amd import Solver, make_workload
+from mlmq_tpds_withl3_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+TIMEOUT = 30.0
+POOL = min(16, os.cpu_count() or 4)
+
+
+def say(msg):
+    print(f"[gpu-configs] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def oracle_rows(row, col, w, sources):
+    with ThreadPoolExecutor(max_workers=POOL) as ex:
+        return list(ex.map(lambda s: O.dijkstra(row, col, w, int(s))[0], sources))
+
+
+def assert_same(got, ref, label):
+    bad = np.flatnonzero(got != ref)
+    assert bad.size == 0, (f"{label}: {bad.size} mismatches, first v={bad[0]} "
+                           f"gpu={got[bad[0]]} cpu={ref[bad[0]]}")
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = Solver(0)
+    yield s
+    s.close()
+
+
+def test_road6m_delta_sweep(solver):
+    say("road6m generate")
+    g, src, spec = make_workload("road6m")
+    row, col, w = g.arrays()
+    ref, work, reached = O.dijkstra(row, col, w, src)
+    solver.load_csr(row, col, w)
+    assert spec["delta_sweep"] == (50_000, 100_000, 200_000, 400_000)
+    for delta in spec["delta_sweep"]:
+        for extra in (dict(), dict(buckets=2, bucket_window=2, l2_batch=16, min_read=16,
+                                   work_blocks=384)):
+            t = solver.solve(src, delta=delta, timeout_s=TIMEOUT, **extra)
+            assert_same(solver.dist(), ref, f"road6m delta={delta} {extra}")
+            say(f"road6m delta={delta} {extra}: device_ms={t['device_ms']:.3f}")
+            assert t["relaxations"] >= work
+    g.close()
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(light_split=1, delta=50, buckets=64, bucket_window=2, l1_pop=64, l2_batch=64,
+         min_read=64, work_blocks=256),
+    dict(light_split=1, delta=50, buckets=64, bucket_window=2, l1_pop=64, l2_batch=64,
+         min_read=64, work_blocks=256, stale_check=3),
+    dict(),                                   # the reference's defaults (delta 200000, 16 buckets)
+    dict(queue="L1V_L2DQ", delta=1000, buckets=8, bucket_window=2),
+])
+def test_rmat22(solver, cfg):
+    if not hasattr(test_rmat22, "graph"):
+        say("rmat22 generate")
+        g, src, _ = make_workload("rmat22")
+        row, col, w = g.arrays()
+        t0 = time.time()
+        test_rmat22.graph = (g, src, O.dijkstra(row, col, w, src))
+        say(f"rmat22 n={g.n} m={g.m} src={src} oracle {time.time() - t0:.1f}s")
+    g, src, (ref, work, reached) = test_rmat22.graph
+    if getattr(test_rmat22, "loaded", None) is not solver or (solver.n, solver.m) != (g.n, g.m):
+        solver.load_graph(g)   # once for all parameter cases (they run back to back)
+        test_rmat22.loaded = solver
+    t = solver.solve(src, timeout_s=TIMEOUT, **cfg)
+    assert_same(solver.dist(), ref, f"rmat22 {cfg}")
+    say(f"rmat22 {cfg}: device_ms={t['device_ms']:.3f} relaxations={t['relaxations']} "
+        f"(E_r {work})")
+
+
+def test_road1m_256_queries_one_launch(solver):
+    say("road1m generate")
+    g, _, spec = make_workload("road1m")
+    row, col, w = g.arrays()
+    sources = np.random.default_rng(12345).integers(0, g.n, size=256)
+    t0 = time.time()
+    refs = oracle_rows(row, col, w, sources)
+    say(f"road1m: 256 oracle rows in {time.time() - t0:.1f}s")
+    solver.load_csr(row, col, w)
+    for cfg in (dict(), dict(batch_waves=16, stale_check=1, prefetch=0, l1_pop=32, min_read=16)):
+        t = solver.solve_batch(sources, timeout_s=60.0, **cfg)
+        assert t["queries"] == 256
+        for i in range(256):
+            assert_same(solver.batch_dist(i), refs[i], f"road1m query {i} (src {sources[i]}) {cfg}")
+        say(f"road1m batch {cfg}: device_ms={t['device_ms']:.2f} "
+            f"p50={t['query_ms_p50']:.2f} p99={t['query_ms_p99']:.2f}")
+    g.close()
+
+
+def test_serving_mix_random_sources():
+    """Poisson arrivals (400 queries/s for 3 s) over three resident graphs at
+    50/30/20 %, sources uniform over each graph's vertices; every result is
+    verified."""
+    names = ("road264k", "road6m", "rmat20")
+    params = {
+        "road264k": dict(delta=200_000),
+        "road6m": dict(delta=400_000, buckets=2, bucket_window=2, l2_batch=16, min_read=16),
+        "rmat20": dict(delta=50, light_split=1, buckets=64, bucket_window=2, l2_batch=64,
+                       min_read=64, l1_pop=64),
+    }
+    rng = np.random.default_rng(2024)
+    rate, duration = 400.0, 3.0
+    gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 1.5) + 16)
+    arrival = np.cumsum(gaps)
+    arrival = arrival[arrival < duration]
+    gsel = rng.choice(3, size=len(arrival), p=[0.5, 0.3, 0.2]).astype(np.int32)
+    server = Server(0, hbm_budget=48 << 30, streams=4, max_batch=64, timeout_s=60.0, verify=1)
+    graphs, srcs = [], np.empty(len(arrival), dtype=np.int32)
+    for gi, name in enumerate(names):
+        g, _, _ = make_workload(name)
+        row, col, w = g.arrays()
+        sel = np.flatnonzero(gsel == gi)
+        srcs[sel] = rng.integers(0, g.n, size=len(sel))
+        gid = server.add_graph(g, **params[name])
+        assert gid == gi
+        t0 = time.time()
+        uniq = np.unique(srcs[sel])
+        for k in range(0, len(uniq), 64):   # oracle rows registered for device verification
+            chunk = uniq[k:k + 64]
+            for s_, r_ in zip(chunk, oracle_rows(row, col, w, chunk)):
+                server.set_expected(gid, int(s_), r_)
+        say(f"serve graph {name}: n={g.n} requests={len(sel)} oracle rows {time.time() - t0:.1f}s")
+        graphs.append((g, row, col, w))
+    # every 8th request also returns its row to the host
+    outs = [np.empty(server.sizes[gsel[i]][0], dtype=np.int32) if i % 8 == 0 else None
+            for i in range(len(arrival))]
+    res, st = server.run(arrival, gsel, srcs, dist_out=outs)
+    say(f"serve: {st['completed']} completed, {st['verified']} verified, "
+        f"{st['completed_qps']:.1f} q/s, p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, "
+        f"mean batch {st['mean_batch']:.1f}")
+    assert st["completed"] == len(arrival) and st["failed"] == 0 and st["rejected"] == 0
+    assert st["verified"] == len(arrival) and st["mismatched"] == 0
+    assert (res["mismatches"] == 0).all() and (res["status"] == 0).all()
+    sample = list(range(0, len(arrival), 8))
+    with ThreadPoolExecutor(max_workers=POOL) as ex:
+        refs = list(ex.map(lambda i: O.dijkstra(*graphs[gsel[i]][1:], int(srcs[i]))[0], sample))
+    for i, ref in zip(sample, refs):
+        assert_same(outs[i], ref, f"served request {i} graph {names[gsel[i]]} src {srcs[i]}")
+    server.close()
+    for g, *_ in graphs:
+        g.close()

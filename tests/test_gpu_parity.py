@@ -277,9 +277,29 @@ def test_errors_and_recovery(solver):
     with pytest.raises(MLMQError) as e:
         solver.solve(0, queue_mode=3)   # L1V_L2V: not on this path
     assert e.value.code == -9
-    # watchdog forced from the host: every wave sees the deadline passed and exits
+    with pytest.raises(MLMQError) as e:
+        solver.solve(0, min_read=0)   # would leave the seed untaken: refused before launch
+    assert e.value.code == -1
+    # a good solve, then one stopped by the watchdog (forced from the host:
+    # every wave sees the deadline passed and exits): the failed solve's
+    # partial distances must not be handed out as a result
+    solve_check(solver, row, col, w, 0, load=False)
     with pytest.raises(MLMQError) as e:
         solver.solve(0, timeout_s=1e-7, delta=100)
     assert e.value.code == -7
+    with pytest.raises(MLMQError) as e:
+        solver.dist()
+    assert e.value.code == -2
     # the next query still runs on a clean workspace
     solve_check(solver, row, col, w, 0, load=False)
+    # the same for a batch: a good batch, a stopped one, no rows to copy
+    solver.solve_batch(np.array([0, 7, 99], dtype=np.int32), timeout_s=TIMEOUT)
+    with pytest.raises(MLMQError) as e:
+        solver.solve_batch(np.array([0, 7, 99], dtype=np.int32), timeout_s=1e-7, delta=100)
+    assert e.value.code == -7
+    with pytest.raises(MLMQError) as e:
+        solver.batch_dist(0)
+    assert e.value.code == -2
+    solver.solve_batch(np.array([5, 6], dtype=np.int32), timeout_s=TIMEOUT)
+    ref, _, _ = O.dijkstra(row, col, w, 6)
+    assert np.array_equal(solver.batch_dist(1), ref)
