@@ -1,0 +1,177 @@
+"""The L2 queue and credit protocol on the CPU (tests/native/queue_model.cpp).
+
+The model runs the device code's rules (bucket choice and clamp, per-bucket
+reservation, CAS publication with a bounded spin, one-snapshot pops with
+persistent claims, window advance, credit return when a wave runs out of
+work, done-first termination check) with each model wave a user-level
+context and every shared-memory access of a lane one step of a seeded
+scheduler, so each seed is one reproducible interleaving.  Checked here:
+
+* no loss, no duplication, exact distances: full solves on random, zero-
+  weight, grid and hub graphs, over many interleavings and queue shapes
+  (one bucket; narrow and wide queues; claims of 1..64; L1 spill of 1..64;
+  a slow termination wave), end with the oracle's distances, every reserved
+  record taken and credited, every slot empty;
+* termination only at quiescence: at the instant the check raises exit, no
+  record is queued or being expanded (ghost count of the model);
+* the two orders the build chose against: reading the reservations before the
+  credits (the reference's order, SS/SSSP/sssp_run.cu:644-661 with
+  cu_delta_queue.cuh:591-600) lets the check exit early, and the window
+  rule from before the fix (a claim on the head bucket holds the window
+  back) stalls the device queue check's scenario;
+* overflow: a ring too small for the work ends every wave with the overflow
+  error instead of losing records.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from graphs_util import grid_csr, hub_csr, random_csr
+from oracle import oracle as O
+
+LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libqueuemodel.so")
+ERR_OVERFLOW = 1
+
+
+@pytest.fixture(scope="module")
+def qm():
+    if not os.path.exists(LIB):
+        raise RuntimeError(f"{LIB} missing: __graft_entry__.build() compiles it")
+    L = ctypes.CDLL(LIB)
+    P, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+    L.qm_solve.argtypes = [ctypes.c_int32, P, P, P, ctypes.c_int32, ctypes.c_int32, P, u64, u64, P, P]
+    L.qm_solve.restype = ctypes.c_int
+    L.qm_publication.argtypes = [u32] * 5 + [P, u64, u64, P]
+    L.qm_publication.restype = ctypes.c_int
+    return L
+
+
+SOLVE_OUT = ("finished", "steps", "err", "exits", "exit_violations", "tail", "taken", "done",
+             "full_slots", "outstanding", "exit", "_")
+
+
+def model_solve(L, g, src, seed, *, buckets=4, window=2, cap_log2=10, delta=50, claim=8,
+                spin=64, pop=32, min_read=8, spill=64, waves=4, advance_rule=0, tails_first=0,
+                slow_leader=1, max_steps=3_000_000):
+    row, col, w = (np.ascontiguousarray(x, dtype=t) for x, t in zip(g, (np.int64, np.int32, np.int32)))
+    n = len(row) - 1
+    cfg = np.array([buckets, window, cap_log2, delta, claim, spin, pop, min_read, spill,
+                    advance_rule, tails_first, slow_leader], dtype=np.uint32)
+    dist = np.zeros(n, dtype=np.uint32)
+    out = np.zeros(12, dtype=np.uint64)
+    rc = L.qm_solve(n, row.ctypes.data, col.ctypes.data, w.ctypes.data, int(src), waves,
+                    cfg.ctypes.data, seed, max_steps, dist.ctypes.data, out.ctypes.data)
+    assert rc == 0
+    return dist.view(np.int32), dict(zip(SOLVE_OUT, (int(x) for x in out)))
+
+
+GRAPHS = {
+    "random": lambda: random_csr(160, 700, seed=1, wmax=100),
+    "zero_weights": lambda: random_csr(140, 600, seed=2, wmax=60, zero_frac=0.3),
+    "grid": lambda: grid_csr(9, 13, seed=3, wmax=200),
+    "hubs": lambda: hub_csr(150, [90, 60], 300, seed=4, wmax=100),
+}
+
+SHAPES = [
+    dict(buckets=1, window=1, claim=8, spill=64, pop=32, min_read=8, waves=4),
+    dict(buckets=4, window=2, claim=2, spill=2, pop=2, min_read=2, waves=4),
+    dict(buckets=4, window=1, claim=64, spill=4, pop=4, min_read=1, waves=3, delta=5,
+         slow_leader=50),
+    dict(buckets=16, window=4, claim=8, spill=4, pop=4, min_read=4, waves=6, delta=10),
+    dict(buckets=64, window=2, claim=1, spill=1, pop=1, min_read=1, waves=3, delta=3),
+]
+
+
+@pytest.mark.parametrize("gname", sorted(GRAPHS))
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_exact_and_quiescent_under_interleavings(qm, gname, shape):
+    g = GRAPHS[gname]()
+    src = 0
+    ref, _, _ = O.dijkstra(*g, src)
+    for seed in range(25):
+        dist, o = model_solve(qm, g, src, seed, **SHAPES[shape])
+        label = f"{gname} shape {shape} seed {seed}: {o}"
+        assert o["finished"] == 1 and o["err"] == 0, label
+        assert o["exits"] == 1 and o["exit_violations"] == 0, label
+        assert o["tail"] == o["taken"] == o["done"], label
+        assert o["full_slots"] == 0 and o["outstanding"] == 0, label
+        assert np.array_equal(dist, ref), label
+
+
+EARLY = dict(buckets=4, window=1, claim=1, spill=4, pop=4, min_read=1, waves=2, delta=5,
+             slow_leader=100)
+
+
+def test_reservations_first_check_exits_early(qm):
+    """The reference's order (tails, then credits) raises exit while records
+    are still queued: the first such interleaving is within a few dozen
+    seeds; the same interleavings with the done-first order never exit early."""
+    g = random_csr(200, 800, seed=3, wmax=100)
+    ref, _, _ = O.dijkstra(*g, 0)
+    early = None
+    for seed in range(600):
+        _, o = model_solve(qm, g, 0, seed, tails_first=1, **EARLY)
+        if o["exit_violations"]:
+            early = seed
+            break
+    assert early is not None, "no early exit found with the tails-first order"
+    for seed in range(early + 1):
+        dist, o = model_solve(qm, g, 0, seed, tails_first=0, **EARLY)
+        assert o["exit_violations"] == 0 and o["finished"] == 1, (seed, o)
+        assert np.array_equal(dist, ref), seed
+
+
+PUB_OUT = ("finished", "steps", "consumed", "bad", "push_fail", "not_once", "tail", "taken")
+
+
+def model_publication(L, seed, advance_rule, *, producers=32, batch=64, consumers=4,
+                      buckets=16, window=4, cap_log2=12, delta=100, claim=64, span=3200, base=0,
+                      max_steps=400_000):
+    cfg = np.array([buckets, window, cap_log2, delta, claim, 1 << 16, 0, 0, 0, advance_rule, 0, base],
+                   dtype=np.uint32)
+    out = np.zeros(8, dtype=np.uint64)
+    rc = L.qm_publication(producers, batch, consumers, 0, span, cfg.ctypes.data, seed, max_steps,
+                          out.ctypes.data)
+    assert rc == 0
+    return dict(zip(PUB_OUT, (int(x) for x in out)))
+
+
+def test_window_rule_before_the_fix_stalls(qm):
+    """The device queue check's scenario (32 producer waves x 64 records, 4
+    consumer waves claiming 64, 16 buckets, window 4): with the old rule some
+    interleavings stop consuming for good (every consumer holds a claim past
+    the head bucket's tail); the current rule consumes every record exactly
+    once in every interleaving."""
+    stalled = []
+    for seed in range(60):
+        o = model_publication(qm, seed, advance_rule=1)
+        if not o["finished"]:
+            assert o["consumed"] < 2048
+            stalled.append(seed)
+    assert stalled, "the old window rule never stalled"
+    for seed in range(60):
+        o = model_publication(qm, seed, advance_rule=0)
+        assert o["finished"] == 1, (seed, o)
+        assert o["consumed"] == o["tail"] == o["taken"] == 2048, (seed, o)
+        assert o["bad"] == o["push_fail"] == o["not_once"] == 0, (seed, o)
+
+
+@pytest.mark.parametrize("base", [0, 7])
+@pytest.mark.parametrize("claim", [1, 8, 64])
+def test_publication_no_loss_no_duplication(qm, base, claim):
+    for seed in range(8):
+        o = model_publication(qm, seed, 0, producers=16, batch=48, consumers=3, claim=claim,
+                              base=base)
+        assert o["finished"] == 1 and o["consumed"] == o["tail"] == o["taken"] == 16 * 48, (seed, o)
+        assert o["bad"] == o["push_fail"] == o["not_once"] == 0, (seed, o)
+
+
+def test_overflow_ends_every_wave_with_the_error(qm):
+    g = grid_csr(20, 20, seed=5, wmax=50)
+    for seed in range(5):
+        _, o = model_solve(qm, g, 0, seed, buckets=1, window=1, cap_log2=3, claim=64, spill=1,
+                           pop=1, min_read=1, waves=3, spin=32)
+        assert o["finished"] == 1, o          # every wave returned
+        assert o["err"] == ERR_OVERFLOW and o["exit"] == 1, o
