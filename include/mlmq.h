@@ -211,6 +211,26 @@ int mlmq_graph_size(const mlmq_ctx *ctx, int32_t *n, int64_t *m);
 /* One query: reset, seed `src`, run the persistent solver to termination.
  * `params` may be NULL (defaults); `timing` may be NULL. */
 int mlmq_solve(mlmq_ctx *ctx, int32_t src, const mlmq_params *params, mlmq_timing *timing);
+/* Ring slots (8 bytes each) of the static device workspace that
+ * mlmq_solve_noalloc uses: buckets x bucket_capacity may not exceed it. */
+#define MLMQ_NOALLOC_RING_SLOTS (1u << 23)
+
+/* mlmq_solve without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The query
+ * is the same as mlmq_solve's on the context's graph (results identical per
+ * vertex); the n distances go to the caller's device buffer `dist_d` (int32,
+ * INT_MAX = unreachable), and the control block and bucket rings live in a
+ * static workspace of the library (64 MiB + 64 KiB per device, held until the
+ * process ends), cleared on every call.  Limit: buckets x bucket_capacity <=
+ * MLMQ_NOALLOC_RING_SLOTS; an explicit bucket_capacity beyond it is refused
+ * with MLMQ_E_INVALID, the automatic one is clamped to it, and overflow
+ * retries stop at it (MLMQ_E_QUEUE_OVERFLOW).  The context's own last result
+ * (mlmq_copy_dist) is left untouched.  Calls that overlap on one device share
+ * the workspace and race: run them one at a time per device. */
+int mlmq_solve_noalloc(mlmq_ctx *ctx, int32_t src, const mlmq_params *params, int32_t *dist_d,
+                       mlmq_timing *timing);
+
 /* Many queries in one persistent launch (no reference counterpart in the
  * single-GPU solver; the reference answers a source list with one solve
  * after another, SSSP/main.cu:213-307).  Each workgroup of

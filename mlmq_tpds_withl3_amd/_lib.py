@@ -21,13 +21,14 @@ ERRORS = {
     -8: "MLMQ_E_INTERNAL", -9: "MLMQ_E_UNSUPPORTED", -10: "MLMQ_E_REJECTED",
 }
 QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1, "L1V_L2DQ": 0}
+NOALLOC_RING_SLOTS = 1 << 23   # MLMQ_NOALLOC_RING_SLOTS
 
 # Every exported symbol the headers declare (checked by the CPU test suite).
 SOLVER_SYMBOLS = [
     "mlmq_params_default", "mlmq_params_check", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
     "mlmq_load_csr", "mlmq_load_gr", "mlmq_attach_device_csr", "mlmq_graph_stats",
     "mlmq_graph_size", "mlmq_solve", "mlmq_copy_dist", "mlmq_dist_device_ptr",
-    "mlmq_device_shape", "mlmq_probe_copy", "mlmq_solve_batch", "mlmq_copy_batch_dist", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
+    "mlmq_device_shape", "mlmq_probe_copy", "mlmq_solve_noalloc", "mlmq_solve_batch", "mlmq_copy_batch_dist", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
 ]
 GRAPH_SYMBOLS = [
     "mlmq_graph_generate_knn", "mlmq_graph_generate_rmat", "mlmq_graph_read_gr",
@@ -211,6 +212,8 @@ def load() -> ctypes.CDLL:
         "mlmq_device_shape": (ctypes.c_int, [P, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                              ctypes.POINTER(i32)]),
         "mlmq_probe_copy": (ctypes.c_int, [P, u64, ctypes.POINTER(ctypes.c_double)]),
+        "mlmq_solve_noalloc": (ctypes.c_int, [P, i32, ctypes.POINTER(Params), P,
+                                              ctypes.POINTER(Timing)]),
         "mlmq_solve_batch": (ctypes.c_int, [P, P, i32, ctypes.POINTER(Params),
                                             ctypes.POINTER(BatchTiming)]),
         "mlmq_copy_batch_dist": (ctypes.c_int, [P, i32, P]),

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -65,12 +66,44 @@ struct mlmq_ctx {
     bool b_solved = false;
     hipGraphExec_t b_graph = nullptr;
     std::vector<uint64_t> b_graph_key;
+    // this device's copy of the static workspace (mlmq_solve_noalloc)
+    uint8_t *scratch = nullptr;
     std::string err;
 };
+
+// Static workspace of mlmq_solve_noalloc: a control block, then
+// MLMQ_NOALLOC_RING_SLOTS ring slots of 8 bytes (64 MiB + 64 KiB in all).
+// Every device the library runs on holds its own copy until the process
+// ends.  Host code never uses the array's name as an address: its device
+// address on each device comes from hipGetSymbolAddress at context creation
+// (scratch_address below) and is kept per device.
+static constexpr uint64_t kScratchCtlBytes = 64ull << 10;
+static constexpr uint64_t kScratchBytes = kScratchCtlBytes + 8ull * MLMQ_NOALLOC_RING_SLOTS;
+static_assert(sizeof(uint32_t) * mlmq::kCtlWords <= kScratchCtlBytes, "control block fits");
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / 8];
 
 namespace {
 
 using clk = std::chrono::steady_clock;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices];
+bool g_scratch_looked_up[kMaxDevices];
+
+// Device address of this device's workspace; `device` must be current.
+// Looked up once per device.  Returns null if the lookup failed.
+uint8_t *scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_looked_up[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = e == hipSuccess ? static_cast<uint8_t *>(ptr) : nullptr;
+        g_scratch_looked_up[device] = true;
+    }
+    return g_scratch_ptr[device];
+}
 
 double ms_since(clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
@@ -398,6 +431,187 @@ int mlmq::ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v) {
     return MLMQ_OK;
 }
 
+namespace {
+
+// Launch shape of one query, shared by mlmq_solve and mlmq_solve_noalloc.
+struct QueryPlan {
+    mlmq_params p;
+    bool slf = true, fifo = false, hot = false;
+    int blocks = 0;
+    uint32_t cap_log2 = 0;   // default ring capacity (slots per bucket, log2)
+};
+
+// Argument and parameter checks, light-edge counts, grid size and the
+// default ring capacity of one query on the context's graph.  Allocates
+// nothing.
+int plan_query(mlmq_ctx *c, int32_t src, const mlmq_params *params, QueryPlan *q) {
+    if (!c->row) return fail(c, MLMQ_E_NOGRAPH, "no graph loaded");
+    if (src < 0 || src >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
+    if (params) q->p = *params;
+    else mlmq_params_default(&q->p);
+    const mlmq_params &p = q->p;
+    {
+        std::string why;
+        const int prc = check_params(p, false, &why);
+        if (prc) return fail(c, prc, why);
+    }
+    q->fifo = p.queue_mode == MLMQ_QUEUE_L1V_L2DQ;
+    q->slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ || q->fifo;
+    q->hot = q->slf && p.l1_blocks != 0;
+    if (p.light_split) {
+        const int lrc = ensure_light_counts(c, p.delta);
+        if (lrc) return lrc;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+
+    int occ = 0;
+    HIP_TRY(c, mlmq::solver_occupancy(q->slf, q->hot, &occ));
+    if (occ < 1) return fail(c, MLMQ_E_HIP, "solver kernel cannot be resident");
+    int blocks = p.work_blocks;
+    if (blocks == 0 && p.blocks_per_cu > 0) {
+        const int per_cu = p.blocks_per_cu < occ ? p.blocks_per_cu : occ;
+        blocks = per_cu * c->cus;
+    } else if (blocks == 0) {
+        // Latency-bound solver: a road-like graph of n vertices keeps only
+        // O(sqrt(n)) records in flight, so more waves mostly add queue
+        // polling.  Measured optimum on MI355X: ~256 waves at 0.26 M
+        // vertices, 512 at 6.3 M.
+        int waves = (int)(c->n / 25000);
+        if (waves < 128) waves = 128;
+        if (waves > 512) waves = 512;
+        blocks = waves / mlmq::kWavesPerBlock;
+    }
+    if (blocks > occ * c->cus) blocks = occ * c->cus;
+    q->blocks = blocks;
+
+    uint64_t cap = (uint64_t)p.bucket_capacity;
+    if (cap == 0) cap = (uint64_t)c->n > 8192 ? (uint64_t)c->n : 8192;
+    q->cap_log2 = ceil_log2(cap);
+    if (q->cap_log2 > 31) return fail(c, MLMQ_E_INVALID, "bucket capacity too large");
+    return MLMQ_OK;
+}
+
+mlmq::SolverLaunch query_launch(const mlmq_ctx *c, const QueryPlan &q, uint32_t *dist, uint32_t *ctl,
+                                uint64_t *slots, uint32_t cap_log2) {
+    const mlmq_params &p = q.p;
+    const double timeout_s = p.timeout_s > 0 ? p.timeout_s : 60.0;
+    mlmq::SolverLaunch L;
+    L.row = c->row;
+    L.col = c->col;
+    L.wgt = c->wgt;
+    L.blocks = c->blocks;
+    L.hubs = c->hubs;
+    L.split = p.hub_split != 0;
+    L.stale_check = (uint32_t)p.stale_check;
+    L.prefetch = p.prefetch ? 1u : 0u;
+    L.defer_heavy = p.defer_heavy ? 1u : 0u;
+    L.strict_window = p.strict_window ? 1u : 0u;
+    L.light_split = p.light_split ? 1u : 0u;
+    L.hot = q.hot;
+    L.dist = dist;
+    L.slots = slots;
+    L.ctl = ctl;
+    L.n = (uint32_t)c->n;
+    L.nbuckets = (uint32_t)p.buckets;
+    L.window = (uint32_t)p.bucket_window;
+    L.cap_log2 = cap_log2;
+    L.delta = p.delta;
+    L.batch = (uint32_t)p.l2_batch;
+    L.spin_limit = mlmq::kSpinLimit;
+    L.min_read = (uint32_t)p.min_read;
+    L.max_sleep = p.idle_backoff > 0 ? (uint32_t)p.idle_backoff : 16u;
+    L.pop_max = (uint32_t)p.l1_pop;
+    L.spill = (uint32_t)p.l1_spill;
+    L.donate_min = (uint32_t)p.l1_donate;
+    L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
+    L.grid_blocks = q.blocks;
+    L.slf = q.slf;
+    L.fifo = q.fifo;
+    L.profile = p.profile != 0;
+    return L;
+}
+
+struct QueryClock {
+    clk::time_point q0, q1, s0, s1;
+};
+
+// Reset and persistent launch of one query, on the context's stream; the
+// control block after the solve is copied into h (kCtlWords words).
+// fill_slots > 0 first fills that many ring slots with empty records.
+int run_query(mlmq_ctx *c, int32_t src, const mlmq::SolverLaunch &L, uint64_t fill_slots, uint32_t *h,
+              QueryClock *t) {
+    t->q0 = clk::now();
+    HIP_TRY(c, hipEventRecord(c->ev_reset, c->stream));
+    if (fill_slots) HIP_TRY(c, mlmq::launch_fill_u64(L.slots, fill_slots, mlmq::kEmpty, c->stream));
+    HIP_TRY(c, mlmq::launch_reset(L.dist, L.n, (uint32_t)src, L.ctl, L.slots, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    t->q1 = clk::now();
+
+    t->s0 = clk::now();
+    HIP_TRY(c, hipEventRecord(c->ev_start, c->stream));
+    HIP_TRY(c, mlmq::launch_solver(L, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_stop, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    t->s1 = clk::now();
+
+    HIP_TRY(c, hipMemcpy(h, L.ctl, sizeof(uint32_t) * mlmq::kCtlWords, hipMemcpyDeviceToHost));
+    return MLMQ_OK;
+}
+
+// Outcome of a finished query from its control block: the device error, or
+// MLMQ_E_INTERNAL unless every reserved record was taken and credited.
+int query_status(mlmq_ctx *c, const uint32_t *h, uint32_t nbuckets) {
+    const uint32_t derr = h[mlmq::kErrOff];
+    if (derr == mlmq::kErrOverflow)
+        return fail(c, MLMQ_E_QUEUE_OVERFLOW, "L2 bucket ring overflow; raise bucket_capacity");
+    if (derr == mlmq::kErrTimeout) return fail(c, MLMQ_E_TIMEOUT, "device watchdog expired");
+    if (derr != mlmq::kErrNone) return fail(c, MLMQ_E_INTERNAL, "unknown device error");
+    uint32_t reserved = 0, taken = 0;
+    for (uint32_t b = 0; b < nbuckets; ++b) {
+        reserved += h[mlmq::kTailOff + b * mlmq::kStride];
+        taken += h[mlmq::kTakenOff + b * mlmq::kStride];
+    }
+    const uint32_t done = h[mlmq::kDoneOff];
+    if (h[mlmq::kExitOff] != 1 || reserved != taken || reserved != done) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "queue accounting mismatch: reserved=%u taken=%u done=%u exit=%u",
+                      reserved, taken, done, h[mlmq::kExitOff]);
+        return fail(c, MLMQ_E_INTERNAL, buf);
+    }
+    return MLMQ_OK;
+}
+
+void fill_timing(mlmq_ctx *c, const uint32_t *h, const QueryClock &t, int blocks, uint32_t cap_log2,
+                 uint64_t retries, mlmq_timing *timing) {
+    std::memset(timing, 0, sizeof(*timing));
+    float dev_ms = 0;
+    (void)hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_stop);
+    timing->solve_ms = ms_since(t.s0, t.s1);
+    timing->device_ms = dev_ms;
+    timing->reset_ms = ms_since(t.q0, t.q1);
+    timing->query_wall_ms = ms_since(t.q0, t.s1);
+    timing->work_blocks = blocks;
+    timing->waves = blocks * mlmq::kWavesPerBlock;
+    timing->bucket_capacity = (int64_t)1 << cap_log2;
+    const uint64_t *st = reinterpret_cast<const uint64_t *>(h + mlmq::kStatsOff);
+    timing->l2_pushes = st[mlmq::kStatL2Push] + 1;  // + the seed
+    timing->l2_pops = st[mlmq::kStatL2Pop];
+    timing->relaxations = st[mlmq::kStatRelax];
+    timing->atomics = st[mlmq::kStatAtomic];
+    timing->improvements = st[mlmq::kStatImprove];
+    timing->stale = st[mlmq::kStatStale];
+    timing->batches = st[mlmq::kStatBatches];
+    timing->idle_polls = st[mlmq::kStatIdle];
+    timing->retries = retries;
+    for (int i = 0; i < 8; ++i) timing->phase_cycles[i] = st[mlmq::kClkL1Pop + i];
+    for (int i = 0; i < 4; ++i) timing->detail_cycles[i] = st[mlmq::kClkOwner + i];
+    timing->detail_cycles[4] = 0;
+    timing->donated = st[mlmq::kStatDonated];
+    timing->hub_chunks = st[mlmq::kStatChunks];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *mlmq_version(void) { return "mlmq-gfx950 0.2"; }
@@ -461,6 +675,8 @@ int mlmq_create(int device, mlmq_ctx **out) {
         mlmq_destroy(c);
         return MLMQ_E_HIP;
     }
+    // (device current: set above)
+    c->scratch = scratch_address(device);
     *out = c;
     return MLMQ_OK;
 }
@@ -594,164 +810,80 @@ int mlmq_device_shape(mlmq_ctx *c, int32_t *blocks_per_cu_max, int32_t *cus, int
 
 int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing *timing) {
     if (!c) return MLMQ_E_INVALID;
-    if (!c->row) return fail(c, MLMQ_E_NOGRAPH, "no graph loaded");
-    if (src < 0 || src >= c->n) return fail(c, MLMQ_E_INVALID, "source out of range");
-    mlmq_params p;
-    if (params) p = *params;
-    else mlmq_params_default(&p);
-    {
-        std::string why;
-        const int prc = check_params(p, false, &why);
-        if (prc) return fail(c, prc, why);
-    }
-    const bool fifo = p.queue_mode == MLMQ_QUEUE_L1V_L2DQ;
-    const bool slf = p.queue_mode == MLMQ_QUEUE_L1SLF_L2DQ || fifo;
-    if (p.light_split) {
-        const int lrc = ensure_light_counts(c, p.delta);
-        if (lrc) return lrc;
-    }
-    HIP_TRY(c, hipSetDevice(c->device));
-
-    int occ = 0;
-    const bool hot = slf && p.l1_blocks != 0;
-    HIP_TRY(c, mlmq::solver_occupancy(slf, hot, &occ));
-    if (occ < 1) return fail(c, MLMQ_E_HIP, "solver kernel cannot be resident");
-    int blocks = p.work_blocks;
-    if (blocks == 0 && p.blocks_per_cu > 0) {
-        const int per_cu = p.blocks_per_cu < occ ? p.blocks_per_cu : occ;
-        blocks = per_cu * c->cus;
-    } else if (blocks == 0) {
-        // Latency-bound solver: a road-like graph of n vertices keeps only
-        // O(sqrt(n)) records in flight, so more waves mostly add queue
-        // polling.  Measured optimum on MI355X: ~256 waves at 0.26 M
-        // vertices, 512 at 6.3 M.
-        int waves = (int)(c->n / 25000);
-        if (waves < 128) waves = 128;
-        if (waves > 512) waves = 512;
-        blocks = waves / mlmq::kWavesPerBlock;
-    }
-    if (blocks > occ * c->cus) blocks = occ * c->cus;
-
-    uint64_t cap = (uint64_t)p.bucket_capacity;
-    if (cap == 0) cap = (uint64_t)c->n > 8192 ? (uint64_t)c->n : 8192;
-    uint32_t cap_log2 = ceil_log2(cap);
-    if (cap_log2 > 31) return fail(c, MLMQ_E_INVALID, "bucket capacity too large");
-
-    int rc = ensure_dist(c);
+    QueryPlan q;
+    int rc = plan_query(c, src, params, &q);
+    if (rc) return rc;
+    rc = ensure_dist(c);
     if (rc) return rc;
     // The reset below overwrites the last solve's distances: until this solve
     // succeeds there is no result to copy out.
     c->solved = false;
 
-    const double timeout_s = p.timeout_s > 0 ? p.timeout_s : 60.0;
+    const uint32_t nb = (uint32_t)q.p.buckets;
+    uint32_t cap_log2 = q.cap_log2;
     uint64_t retries = 0;
     for (;;) {
-        rc = ensure_queue(c, (uint32_t)p.buckets, cap_log2);
+        rc = ensure_queue(c, nb, cap_log2);
         if (rc) return rc;
-        const auto q0 = clk::now();
-        HIP_TRY(c, hipEventRecord(c->ev_reset, c->stream));
-        HIP_TRY(c, mlmq::launch_reset(c->dist, (uint32_t)c->n, (uint32_t)src, c->ctl, c->slots, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        const auto q1 = clk::now();
-
-        mlmq::SolverLaunch L;
-        L.row = c->row;
-        L.col = c->col;
-        L.wgt = c->wgt;
-        L.blocks = c->blocks;
-        L.hubs = c->hubs;
-        L.split = p.hub_split != 0;
-        L.stale_check = (uint32_t)p.stale_check;
-        L.prefetch = p.prefetch ? 1u : 0u;
-        L.defer_heavy = p.defer_heavy ? 1u : 0u;
-        L.strict_window = p.strict_window ? 1u : 0u;
-        L.light_split = p.light_split ? 1u : 0u;
-        L.hot = hot;
-        L.dist = c->dist;
-        L.slots = c->slots;
-        L.ctl = c->ctl;
-        L.n = (uint32_t)c->n;
-        L.nbuckets = (uint32_t)p.buckets;
-        L.window = (uint32_t)p.bucket_window;
-        L.cap_log2 = cap_log2;
-        L.delta = p.delta;
-        L.batch = (uint32_t)p.l2_batch;
-        L.spin_limit = mlmq::kSpinLimit;
-        L.min_read = (uint32_t)p.min_read;
-        L.max_sleep = p.idle_backoff > 0 ? (uint32_t)p.idle_backoff : 16u;
-        L.pop_max = (uint32_t)p.l1_pop;
-        L.spill = (uint32_t)p.l1_spill;
-        L.donate_min = (uint32_t)p.l1_donate;
-        L.timeout_ticks = (uint64_t)(timeout_s * (double)c->wall_khz * 1000.0);
-        L.grid_blocks = blocks;
-        L.slf = slf;
-        L.fifo = fifo;
-        L.profile = p.profile != 0;
-
-        const auto s0 = clk::now();
-        HIP_TRY(c, hipEventRecord(c->ev_start, c->stream));
-        HIP_TRY(c, mlmq::launch_solver(L, c->stream));
-        HIP_TRY(c, hipEventRecord(c->ev_stop, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        const auto s1 = clk::now();
-
-        HIP_TRY(c, hipMemcpy(c->ctl_host.data(), c->ctl, sizeof(uint32_t) * mlmq::kCtlWords,
-                             hipMemcpyDeviceToHost));
-        const uint32_t *h = c->ctl_host.data();
-        const uint32_t derr = h[mlmq::kErrOff];
-        uint32_t reserved = 0, taken = 0;
-        for (int b = 0; b < p.buckets; ++b) {
-            reserved += h[mlmq::kTailOff + b * mlmq::kStride];
-            taken += h[mlmq::kTakenOff + b * mlmq::kStride];
+        const mlmq::SolverLaunch L = query_launch(c, q, c->dist, c->ctl, c->slots, cap_log2);
+        QueryClock t;
+        rc = run_query(c, src, L, 0, c->ctl_host.data(), &t);
+        if (rc) return rc;
+        rc = query_status(c, c->ctl_host.data(), nb);
+        if (rc) c->slots_dirty = true;
+        if (rc == MLMQ_E_QUEUE_OVERFLOW && q.p.auto_retry && retries < 4 && cap_log2 < 31) {
+            ++retries;
+            ++cap_log2;
+            continue;
         }
-        const uint32_t done = h[mlmq::kDoneOff];
-        if (derr != mlmq::kErrNone) c->slots_dirty = true;
-        if (derr == mlmq::kErrOverflow) {
-            if (p.auto_retry && retries < 4 && cap_log2 < 31) {
-                ++retries;
-                ++cap_log2;
-                continue;
-            }
-            return fail(c, MLMQ_E_QUEUE_OVERFLOW, "L2 bucket ring overflow; raise bucket_capacity");
-        }
-        if (derr == mlmq::kErrTimeout) return fail(c, MLMQ_E_TIMEOUT, "device watchdog expired");
-        if (derr != mlmq::kErrNone) return fail(c, MLMQ_E_INTERNAL, "unknown device error");
-        if (h[mlmq::kExitOff] != 1 || reserved != taken || reserved != done) {
-            c->slots_dirty = true;
-            char buf[160];
-            std::snprintf(buf, sizeof buf, "queue accounting mismatch: reserved=%u taken=%u done=%u exit=%u",
-                          reserved, taken, done, h[mlmq::kExitOff]);
-            return fail(c, MLMQ_E_INTERNAL, buf);
-        }
+        if (rc) return rc;
         c->solved = true;
-        if (timing) {
-            std::memset(timing, 0, sizeof(*timing));
-            float dev_ms = 0, reset_ms = 0;
-            (void)hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_stop);
-            (void)hipEventElapsedTime(&reset_ms, c->ev_reset, c->ev_start);
-            timing->solve_ms = ms_since(s0, s1);
-            timing->device_ms = dev_ms;
-            timing->reset_ms = ms_since(q0, q1);
-            timing->query_wall_ms = ms_since(q0, s1);
-            timing->work_blocks = blocks;
-            timing->waves = blocks * mlmq::kWavesPerBlock;
-            timing->bucket_capacity = (int64_t)1 << cap_log2;
-            const uint64_t *st = reinterpret_cast<const uint64_t *>(h + mlmq::kStatsOff);
-            timing->l2_pushes = st[mlmq::kStatL2Push] + 1;  // + the seed
-            timing->l2_pops = st[mlmq::kStatL2Pop];
-            timing->relaxations = st[mlmq::kStatRelax];
-            timing->atomics = st[mlmq::kStatAtomic];
-            timing->improvements = st[mlmq::kStatImprove];
-            timing->stale = st[mlmq::kStatStale];
-            timing->batches = st[mlmq::kStatBatches];
-            timing->idle_polls = st[mlmq::kStatIdle];
-            timing->retries = retries;
-            for (int i = 0; i < 8; ++i) timing->phase_cycles[i] = st[mlmq::kClkL1Pop + i];
-            for (int i = 0; i < 4; ++i) timing->detail_cycles[i] = st[mlmq::kClkOwner + i];
-            timing->detail_cycles[4] = 0;
-            timing->donated = st[mlmq::kStatDonated];
-            timing->hub_chunks = st[mlmq::kStatChunks];
+        if (timing) fill_timing(c, c->ctl_host.data(), t, q.blocks, cap_log2, retries, timing);
+        return MLMQ_OK;
+    }
+}
+
+// The same query as mlmq_solve, with the control block and the bucket rings
+// in this file's static device workspace and the distances in the caller's
+// buffer: nothing is allocated.  The workspace is filled with empty slots and
+// the control block cleared on every call, so no state is carried between
+// calls; calls that overlap on one device share the workspace and race.
+int mlmq_solve_noalloc(mlmq_ctx *c, int32_t src, const mlmq_params *params, int32_t *dist_d,
+                       mlmq_timing *timing) {
+    if (!c) return MLMQ_E_INVALID;
+    if (!dist_d) return fail(c, MLMQ_E_INVALID, "null distance buffer");
+    QueryPlan q;
+    int rc = plan_query(c, src, params, &q);
+    if (rc) return rc;
+    if (!c->scratch) return fail(c, MLMQ_E_HIP, "static workspace address unavailable on this device");
+    const uint64_t limit = MLMQ_NOALLOC_RING_SLOTS;
+    const uint32_t nb = (uint32_t)q.p.buckets;
+    uint32_t cap_log2 = q.cap_log2;
+    if (((uint64_t)nb << cap_log2) > limit) {
+        if (q.p.bucket_capacity > 0)
+            return fail(c, MLMQ_E_INVALID,
+                        "buckets x bucket_capacity exceeds the static workspace (MLMQ_NOALLOC_RING_SLOTS)");
+        while (((uint64_t)nb << cap_log2) > limit) --cap_log2;
+    }
+    uint32_t *ctl = reinterpret_cast<uint32_t *>(c->scratch);
+    uint64_t *slots = reinterpret_cast<uint64_t *>(c->scratch + kScratchCtlBytes);
+    uint32_t h[mlmq::kCtlWords];
+    uint64_t retries = 0;
+    for (;;) {
+        const mlmq::SolverLaunch L = query_launch(c, q, reinterpret_cast<uint32_t *>(dist_d), ctl, slots,
+                                                  cap_log2);
+        QueryClock t;
+        rc = run_query(c, src, L, (uint64_t)nb << cap_log2, h, &t);
+        if (rc) return rc;
+        rc = query_status(c, h, nb);
+        if (rc == MLMQ_E_QUEUE_OVERFLOW && q.p.auto_retry && retries < 4 &&
+            ((uint64_t)nb << (cap_log2 + 1)) <= limit) {
+            ++retries;
+            ++cap_log2;
+            continue;
         }
+        if (rc) return rc;
+        if (timing) fill_timing(c, h, t, q.blocks, cap_log2, retries, timing);
         return MLMQ_OK;
     }
 }

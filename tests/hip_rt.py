@@ -41,3 +41,18 @@ class DeviceArray:
         if self.ptr:
             hip().hipFree(self.ptr)
             self.ptr = ctypes.c_void_p()
+
+
+def mem_get_info() -> tuple[int, int]:
+    """(free, total) device bytes of the current device."""
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    L = hip()
+    L.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    assert L.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value, total.value
+
+
+def to_host(arr: DeviceArray, dtype, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=dtype)
+    assert hip().hipMemcpy(out.ctypes.data, arr.ptr, out.nbytes, D2H) == 0
+    return out
