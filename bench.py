@@ -43,6 +43,87 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+# ---- HBM traffic of the timed point (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) ----
+
+TRAFFIC_KEYS = ("workload", "queue", "delta", "buckets", "bucket_window", "l2_batch", "min_read",
+                "l1_pop", "l1_donate", "light_split", "stale_check", "prefetch", "work_blocks")
+
+
+def traffic_key(workload, queue, delta, args, work_blocks):
+    """Canonical text of every setting that changes the persistent kernel's
+    memory traffic: a traffic figure is attached to a bench line only when it
+    was measured at exactly this point."""
+    cfg = {"workload": workload, "queue": queue, "delta": int(delta), "buckets": args.buckets,
+           "bucket_window": args.window, "l2_batch": args.l2_batch, "min_read": args.min_read,
+           "l1_pop": args.l1_pop, "l1_donate": args.l1_donate, "light_split": args.light_split,
+           "stale_check": args.stale_check, "prefetch": args.prefetch,
+           "work_blocks": int(work_blocks)}
+    return json.dumps({k: cfg[k] for k in TRAFFIC_KEYS}, sort_keys=True)
+
+
+def load_traffic(path, key):
+    try:
+        tj = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    entry = tj.get(key)
+    return entry["bytes_per_launch"] if entry else None
+
+
+def _counter_means(d, counter, kernel):
+    """Mean over dispatches of `kernel` of `counter` (summed per dispatch) in
+    the rocprofv3 counter_collection CSVs under directory d."""
+    import csv
+    import glob
+    per = {}
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            if r.get("Counter_Name") != counter or kernel not in r.get("Kernel_Name", ""):
+                continue
+            k = (f, r.get("Dispatch_Id") or r.get("Correlation_Id"))
+            per[k] = per.get(k, 0.0) + float(r["Counter_Value"])
+    if not per:
+        raise SystemExit(f"no {counter} rows of {kernel} under {d}")
+    return sum(per.values()) / len(per), len(per)
+
+
+def traffic_update(args):
+    """Bytes per launch of the persistent kernel from two counter runs of the
+    bench, each scaled by what the same counter reads for a plain copy kernel
+    of known size, stored under the bench line's traffic key."""
+    line_path, fdir, wdir, cfdir, cwdir = args.traffic_update
+    line = None
+    for ln in open(line_path):
+        ln = ln.strip()
+        if ln.startswith("{") and '"roofline"' in ln:
+            line = json.loads(ln)
+    if line is None:
+        raise SystemExit(f"no bench line in {line_path}")
+    key = line["roofline"]["traffic_key"]
+    kern = "sssp_persistent_kernel"
+    fetch, nf = _counter_means(fdir, "FETCH_SIZE", kern)
+    write, nw = _counter_means(wdir, "WRITE_SIZE", kern)
+    cfetch, _ = _counter_means(cfdir, "FETCH_SIZE", "copy_u32_kernel")
+    cwrite, _ = _counter_means(cwdir, "WRITE_SIZE", "copy_u32_kernel")
+    fscale = args.copy_bytes / 1024.0 / cfetch
+    wscale = args.copy_bytes / 1024.0 / cwrite
+    fb, wb = fetch * 1024.0 * fscale, write * 1024.0 * wscale
+    try:
+        tj = json.load(open(args.traffic_json))
+    except (OSError, ValueError):
+        tj = {}
+    tj[key] = {"bytes_per_launch": int(round(fb + wb)), "fetch_bytes": int(round(fb)),
+               "write_bytes": int(round(wb)), "fetch_kib_raw": fetch, "write_kib_raw": write,
+               "dispatches": [nf, nw], "fetch_scale": fscale, "write_scale": wscale,
+               "device_ms_of_line": line["device_ms"]["mean"],
+               "note": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch of {kern}, scaled "
+                       f"by the same counters on a {args.copy_bytes}-byte copy kernel"}
+    with open(args.traffic_json, "w") as f:
+        json.dump(tj, f, indent=1, sort_keys=True)
+    print(json.dumps({"key": key, **tj[key]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,8 +150,23 @@ def main():
     ap.add_argument("--p99-bound-ms", type=float, default=1000.0,
                     help="serve mode: value = best completed rate with p99 under this bound")
     ap.add_argument("--tune", type=int, default=1, help="batch/serve modes: run the autotuner")
+    ap.add_argument("--tune-single", type=int, default=0,
+                    help="single mode: tune on --tune-sources other sources instead of timing "
+                         "the workload's committed bench point")
+    ap.add_argument("--tune-sources", type=int, default=3)
+    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--traffic-update", nargs=5, default=None,
+                    metavar=("BENCH_LINE", "FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="write the traffic entry of a profiled bench line into --traffic-json "
+                         "from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE output directories of the "
+                         "bench run and of a --calibrate-copy run")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
+                    help="--traffic-update: size of the calibration copy")
+    ap.add_argument("--delta-sweep", type=int, default=1,
+                    help="single mode: time the workload's delta sweep at the timed point's other "
+                         "settings (0: only the timed point runs, e.g. under a counter profiler)")
     ap.add_argument("--extended-sweep", type=int, default=1,
-                    help="single mode: also report the best delta outside the configured sweep")
+                    help="single mode: also time deltas 2M and 20M (outside the configured sweep)")
     ap.add_argument("--batch-waves", type=int, default=8)
     ap.add_argument("--source-seed", type=int, default=12345)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -107,6 +203,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.traffic_update:
+        return traffic_update(args)
+
     import numpy as np
     from mlmq_tpds_withl3_amd import Solver, make_workload, default_params
 
@@ -142,32 +241,47 @@ def main():
     solver = Solver(0)
     solver.load_csr(row, col, w)
     shape = solver.device_shape()
-    sweep = {}
-    extended = None
+    tuned_on = None
     if args.delta is not None:
         delta = args.delta
-        params = make_params(delta)
     else:
-        # The workload's delta sweep crossed with a small grid of L1 pop size,
-        # L2 top-up threshold and grid size (untimed, best of two solves each).
-        from mlmq_tpds_withl3_amd.autotune import tune_single
-        tuned = tune_single(solver, src, spec["delta_sweep"], grid=spec.get("tune_grid"))
-        for prm, ms in tuned["table"]:
-            sweep[prm["delta"]] = min(ms, sweep.get(prm["delta"], ms))
-        best = tuned["params"]
-        delta = best["delta"]
-        args.l1_pop, args.min_read, args.work_blocks = best["l1_pop"], best["min_read"], best["work_blocks"]
-        args.l2_batch = best.get("l2_batch", args.l2_batch)
-        args.buckets = best.get("buckets", args.buckets)
-        args.window = best.get("bucket_window", args.window)
-        args.light_split = best.get("light_split", args.light_split)
-        args.stale_check = best.get("stale_check", args.stale_check)
-        params = make_params(delta)
-        log(f"tuned over delta {list(spec['delta_sweep'])}: {best} {tuned['device_ms']:.3f} ms")
-        if args.extended_sweep:
-            ext = tune_single(solver, src, (2_000_000, 20_000_000), grid=spec.get("tune_grid"))
-            extended = {"params": ext["params"], "device_ms": round(ext["device_ms"], 4)}
-            log(f"outside the configured sweep: {extended}")
+        point = dict(spec.get("bench_point") or {})
+        if args.tune_single:
+            # Tune on other sources than the timed one (seeded, uniform over
+            # the vertices), so the timed point is not fitted to the measured
+            # query: the workload's delta sweep crossed with its grid.
+            from mlmq_tpds_withl3_amd.autotune import tune_single
+            rng = np.random.default_rng(args.source_seed)
+            tsrc = [int(x) for x in rng.integers(0, g.n, size=args.tune_sources + 1) if x != src]
+            tsrc = tsrc[:args.tune_sources]
+            tuned = tune_single(solver, tsrc, spec["delta_sweep"], grid=spec.get("tune_grid"),
+                                log=log if args.verbose else None)
+            point = tuned["params"]
+            tuned_on = {"sources": tsrc, "device_ms": round(tuned["device_ms"], 4), "params": point}
+            log(f"tuned on sources {tsrc}: {point} {tuned['device_ms']:.3f} ms")
+        delta = point.get("delta", spec["delta"])
+        for key, attr in (("l1_pop", "l1_pop"), ("min_read", "min_read"),
+                          ("work_blocks", "work_blocks"), ("l2_batch", "l2_batch"),
+                          ("buckets", "buckets"), ("bucket_window", "window"),
+                          ("light_split", "light_split"), ("stale_check", "stale_check"),
+                          ("prefetch", "prefetch")):
+            if key in point:
+                setattr(args, attr, point[key])
+    params = make_params(delta)
+
+    # The workload's delta sweep at the timed point's other settings, on the
+    # timed source: reported, not used to choose the point.
+    def sweep_ms(deltas):
+        table = {}
+        for d in deltas:
+            runs = sorted(solver.solve(src, make_params(int(d)))["device_ms"] for _ in range(3))
+            table[str(int(d))] = round(runs[1], 4)
+        return table
+    sweep = sweep_ms(spec["delta_sweep"]) if args.delta_sweep else None
+    log(f"delta sweep (median of 3, device ms): {sweep}")
+    extended = sweep_ms((2_000_000, 20_000_000)) if args.extended_sweep and args.delta_sweep else None
+    if extended:
+        log(f"outside the configured sweep: {extended}")
     for i in range(args.warmup):
         t = solver.solve(src, params)
         log(f"warmup {i}: solve_ms={t['solve_ms']:.3f} device_ms={t['device_ms']:.3f}")
@@ -221,15 +335,9 @@ def main():
     b_alg = 16 * v_r + 12 * e_r
     dev_avg_s = statistics.mean(dev_ms) / 1e3
     achieved = b_alg / dev_avg_s / 1e9
-    traffic = None
-    if os.path.exists(args.traffic_json):
-        try:
-            tj = json.load(open(args.traffic_json))
-            key = f"{args.workload}:{args.queue}"
-            if key in tj:
-                traffic = tj[key]["bytes_per_launch"]
-        except (OSError, ValueError, KeyError):
-            traffic = None
+    last = timings[-1]
+    tkey = traffic_key(args.workload, args.queue, delta, args, last["work_blocks"])
+    traffic = load_traffic(args.traffic_json, tkey)
     last = timings[-1]
     out = {
         "metric": METRIC,
@@ -251,10 +359,13 @@ def main():
             "weight_scale": spec.get("weight_scale_applied"),
             "seed": spec.get("seed"),
             "queue": args.queue, "delta": delta,
-            "delta_sweep_device_ms": {str(k): round(v, 4) for k, v in sweep.items()},
-            "outside_sweep_best": extended,
+            "delta_sweep_device_ms": sweep,
+            "outside_sweep_device_ms": extended,
+            "point": "tuned on other sources" if tuned_on else
+                     ("command line" if args.delta is not None else "committed bench point"),
+            "tuned_on": tuned_on,
             "buckets": args.buckets, "bucket_window": args.window, "light_split": args.light_split,
-            "stale_check": args.stale_check,
+            "stale_check": args.stale_check, "prefetch": args.prefetch,
             "l2_batch": args.l2_batch, "min_read": args.min_read, "l1_pop": args.l1_pop, "l1_donate": args.l1_donate,
             "work_blocks": last["work_blocks"], "waves": last["waves"],
             "cus": shape["cus"], "blocks_per_cu_max": shape["blocks_per_cu_max"],
@@ -281,6 +392,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 8),
             "traffic": traffic,
+            "traffic_key": tkey,
             "kernel": "sssp_persistent_kernel",
             "bytes_per_launch": b_alg,
         },

@@ -82,7 +82,7 @@ typedef struct mlmq_params {
     int32_t  profile;         /* 1: per-phase clock build of the kernel (mlmq_timing.phase_cycles) */
     int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 32 */
     int32_t  l1_spill;        /* L1 holding more records than this after a push spills to L2;
-                                 1..64, default 64 */
+                                 1..l1_ring - 448 (64 with the default ring), default 64 */
     int32_t  l1_donate;       /* while some waves are idle, a wave whose L1 holds more than this
                                  moves the back half to L2; 0 = never (default) */
     int32_t  batch_waves;     /* mlmq_solve_batch: waves per query (workgroup size), 4, 8 or 16;
@@ -118,7 +118,14 @@ typedef struct mlmq_params {
                                  heavy-phase record, one bucket later, that relaxes the edges of
                                  weight >= delta; needs a graph loaded through mlmq_load_csr /
                                  mlmq_load_gr (adjacency sorted by weight); 0 (default) */
-    int32_t  reserved[1];
+    int32_t  csr_stream;      /* 1: CSR columns and weights of vertices with more than 7 out-edges
+                                 are read with non-temporal loads, so streaming a hub's edge list
+                                 does not evict the distance lines its relaxations test; 0 (default) */
+    int32_t  l1_ring;         /* L1 ring size per wave in records (the LDS segment: 8 bytes each):
+                                 0 or 512 (default); mlmq_solve_batch also takes 1024, which lets
+                                 l1_spill go up to 255 (the spill threshold must leave room for one
+                                 batch's pushes: l1_spill <= l1_ring - 448) */
+    int32_t  reserved[3];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */
@@ -242,6 +249,28 @@ int mlmq_solve_batch(mlmq_ctx *ctx, const int32_t *sources, int32_t nq,
                      const mlmq_params *params, mlmq_batch_timing *timing);
 /* Copy the n distances of query `query` of the last batch (caller-owned). */
 int mlmq_copy_batch_dist(mlmq_ctx *ctx, int32_t query, int32_t *host_out);
+
+/* Queue counts of the last mlmq_solve, per bucket: the reference's capacity
+ * and conservation audit (L2_CAPACITY at queue setup,
+ * core/cu_delta_queue/cu_delta_queue.cuh:165; CAPACITY_BUCKET and
+ * CAPACITY_QUERY per query, SS/SSSP/supplement_capacity.h:2-13 of the
+ * diagnostic build; L2_FINAL, SSSP/sssp_run.cu:10900-10926).  After every
+ * successful solve writes[b] == reads[b] for every bucket and their sum ==
+ * completed.  A bucket whose writes exceed its capacity reused ring slots:
+ * each reuse waited until the earlier record was taken (slot-level
+ * publication), so nothing was overwritten. */
+typedef struct mlmq_queue_counts {
+    int32_t  buckets;
+    int32_t  window;
+    int64_t  capacity;        /* ring slots per bucket */
+    uint32_t writes[64];      /* records reserved in bucket b (the reference's write_reserve) */
+    uint32_t reads[64];       /* records taken from bucket b (bucket_read_done) */
+    uint32_t completed;       /* credits returned (read_done) */
+    uint32_t epoch;           /* window advances (the reference's first_pos) */
+    uint32_t wrapped;         /* buckets whose writes exceeded the capacity */
+    uint32_t reserved[5];
+} mlmq_queue_counts;
+int mlmq_last_queue_counts(const mlmq_ctx *ctx, mlmq_queue_counts *out);
 
 /* Copy the n distances of the last solve into host memory (caller-owned). */
 int mlmq_copy_dist(mlmq_ctx *ctx, int32_t *host_out);

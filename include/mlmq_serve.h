@@ -19,11 +19,15 @@
  *     Slots stay with their graph while free (their queue partition is then
  *     already clean) and are returned to the arena when another graph needs
  *     the bytes.
- *   - Batching: whenever one of config.streams launch lanes is free, the
- *     oldest waiting request's graph is served: up to config.max_batch of its
- *     waiting requests that can be admitted go into one launch.  Lanes run
- *     concurrently on the device, so requests that arrive while others run are
- *     admitted as lanes and bytes free up (continuous batching).
+ *   - Batching: whenever one of config.streams launch lanes is free, waiting
+ *     requests go into one launch in arrival order, up to config.max_batch,
+ *     while each can be admitted.  A launch mixes requests of different
+ *     graphs and sizes (each request carries its own graph and settings);
+ *     the oldest waiting request's graph fixes the launch's workgroup width
+ *     (mlmq_serve_graph_params.waves), and requests of graphs with another
+ *     width wait for a later launch.  Lanes run concurrently on the device,
+ *     so requests that arrive while others run are admitted as lanes and
+ *     bytes free up (continuous batching).
  *   - Each request runs on `workgroups` workgroups of `waves` wave64 that
  *     share its queue partition; distances are exact (the same per vertex as
  *     a CPU Dijkstra), whatever else runs beside it.

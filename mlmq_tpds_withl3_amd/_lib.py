@@ -28,7 +28,7 @@ SOLVER_SYMBOLS = [
     "mlmq_params_default", "mlmq_params_check", "mlmq_create", "mlmq_destroy", "mlmq_last_error", "mlmq_version",
     "mlmq_load_csr", "mlmq_load_gr", "mlmq_attach_device_csr", "mlmq_graph_stats",
     "mlmq_graph_size", "mlmq_solve", "mlmq_copy_dist", "mlmq_dist_device_ptr",
-    "mlmq_device_shape", "mlmq_probe_copy", "mlmq_solve_noalloc", "mlmq_solve_batch", "mlmq_copy_batch_dist", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
+    "mlmq_device_shape", "mlmq_probe_copy", "mlmq_solve_noalloc", "mlmq_last_queue_counts", "mlmq_solve_batch", "mlmq_copy_batch_dist", "sssp_init", "sssp_run_adaptive", "sssp_cp_data", "sssp_release",
 ]
 GRAPH_SYMBOLS = [
     "mlmq_graph_generate_knn", "mlmq_graph_generate_rmat", "mlmq_graph_read_gr",
@@ -65,7 +65,8 @@ class Params(ctypes.Structure):
         ("stale_check", ctypes.c_int32), ("prefetch", ctypes.c_int32),
         ("l1_blocks", ctypes.c_int32), ("idle_backoff", ctypes.c_int32),
         ("defer_heavy", ctypes.c_int32), ("strict_window", ctypes.c_int32),
-        ("light_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
+        ("light_split", ctypes.c_int32), ("csr_stream", ctypes.c_int32),
+        ("l1_ring", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -110,6 +111,21 @@ class BatchTiming(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class QueueCounts(ctypes.Structure):
+    _fields_ = [
+        ("buckets", ctypes.c_int32), ("window", ctypes.c_int32), ("capacity", ctypes.c_int64),
+        ("writes", ctypes.c_uint32 * 64), ("reads", ctypes.c_uint32 * 64),
+        ("completed", ctypes.c_uint32), ("epoch", ctypes.c_uint32), ("wrapped", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 5),
+    ]
+
+    def as_dict(self) -> dict:
+        nb = self.buckets
+        return {"buckets": nb, "window": self.window, "capacity": self.capacity,
+                "writes": list(self.writes)[:nb], "reads": list(self.reads)[:nb],
+                "completed": self.completed, "epoch": self.epoch, "wrapped": self.wrapped}
 
 
 class GraphInfo(ctypes.Structure):
@@ -214,6 +230,7 @@ def load() -> ctypes.CDLL:
         "mlmq_probe_copy": (ctypes.c_int, [P, u64, ctypes.POINTER(ctypes.c_double)]),
         "mlmq_solve_noalloc": (ctypes.c_int, [P, i32, ctypes.POINTER(Params), P,
                                               ctypes.POINTER(Timing)]),
+        "mlmq_last_queue_counts": (ctypes.c_int, [P, ctypes.POINTER(QueueCounts)]),
         "mlmq_solve_batch": (ctypes.c_int, [P, P, i32, ctypes.POINTER(Params),
                                             ctypes.POINTER(BatchTiming)]),
         "mlmq_copy_batch_dist": (ctypes.c_int, [P, i32, P]),

@@ -26,17 +26,21 @@ def default_delta(row, w) -> int:
     return max(1, int(8 * float(w.mean()))) if len(w) else 1
 
 
-def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, confirm: int = 7,
+def tune_single(solver, src, deltas, grid=None, repeats: int = 2, confirm: int = 7,
                 log=None) -> dict:
-    """Grid search for single queries: every delta in `deltas` crossed with
-    `grid` (a list of parameter dicts, default: L1 pop size / L2 top-up
-    threshold pairs, bucket counts and three grid sizes), then the late and L2-cached stale
-    checks at the four best points.  Each point is the best device time of
-    `repeats` solves; the four best are then timed `confirm` times more and
-    the lowest median wins (a best-of-two is often a lucky draw).  Returns the
-    best point, its median and the whole table."""
+    """Grid search for single queries from `src` (one source or a list of
+    sources: a point's time is then the mean over them): every delta in
+    `deltas` crossed with `grid` (a list of parameter dicts, default: L1 pop
+    size / L2 top-up threshold pairs, bucket counts and three grid sizes),
+    then the late and L2-cached stale checks at the four best points.  Each
+    point is the best device time of `repeats` solves per source; the four
+    best are then timed `confirm` times more and the lowest median wins (a
+    best-of-two is often a lucky draw).  Returns the best point, its median
+    and the whole table.  Tune on other sources than the one a benchmark
+    then times, so the point is not fitted to the measured query."""
     from .solver import default_params
 
+    sources = [int(s) for s in (src if np.ndim(src) else [src])]
     if grid is None:
         grid = [dict(l1_pop=pop, min_read=mr, l2_batch=lb, work_blocks=wb, buckets=nb,
                      bucket_window=min(4, nb))
@@ -46,7 +50,8 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, confirm: 
 
     def measure(params):
         p = default_params(timeout_s=30.0, **params)
-        ms = min(solver.solve(src, p)["device_ms"] for _ in range(repeats))
+        ms = float(np.mean([min(solver.solve(s, p)["device_ms"] for _ in range(repeats))
+                            for s in sources]))
         table.append((params, ms))
         if log:
             log(f"tune single: {params} -> {ms:.3f} ms")
@@ -62,12 +67,16 @@ def tune_single(solver, src: int, deltas, grid=None, repeats: int = 2, confirm: 
     finals = []
     for bp, _ in sorted(table, key=lambda t: t[1])[:4]:
         p = default_params(timeout_s=30.0, **bp)
-        runs = sorted(solver.solve(src, p)["device_ms"] for _ in range(max(1, confirm)))
-        finals.append((bp, runs[len(runs) // 2]))
+        meds = []
+        for s in sources:
+            runs = sorted(solver.solve(s, p)["device_ms"] for _ in range(max(1, confirm)))
+            meds.append(runs[len(runs) // 2])
+        finals.append((bp, float(np.mean(meds))))
         if log:
             log(f"tune single, confirm: {bp} -> median {finals[-1][1]:.3f} ms")
     best = min(finals, key=lambda t: t[1])
-    return {"params": best[0], "device_ms": best[1], "table": table, "confirmed": finals}
+    return {"params": best[0], "device_ms": best[1], "table": table, "confirmed": finals,
+            "sources": sources}
 
 
 def tune_serving(server, gid: int, sources, base: dict | None = None, burst: int = 64,
@@ -169,11 +178,17 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
     # bandwidth than they save), then delta, the bucket count and the queue
     # sizes once more at the point reached: their best values move with the
     # shape.
+    # The L1 ring (the LDS segment of each wave) and its spill threshold: a
+    # 1024-record ring lets a wave keep up to 255 records before spilling to
+    # L2 (a 512-record ring at most 64).
     space = [("batch_waves", [4, 8, 16]), ("delta", None),
              ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32]),
+             ("l1_ring", [512, 1024]), ("l1_spill", [64, 128, 255]),
              ("prefetch", [0, 1]), ("stale_check", [1, 3, 2]),
              ("delta", None), ("buckets", [8, 16, 32]),
              ("l1_pop", [16, 32, 64]), ("min_read", [8, 16, 32])]
+    cur.setdefault("l1_ring", 512)
+    cur.setdefault("l1_spill", 64)
     for key, values in space:
         if values is None:           # delta: a grid around the current value
             values = _delta_grid(cur["delta"])
@@ -185,6 +200,10 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
                 cand["bucket_window"] = min(4, v)
             if key == "min_read":
                 cand["l2_batch"] = max(8, v)
+            if key == "l1_ring" and v == 512:
+                cand["l1_spill"] = min(cand["l1_spill"], 64)
+            if cand["l1_spill"] > cand["l1_ring"] - 448:
+                continue             # a batch's pushes would not fit above the threshold
             t = score(cand)
             trail.append((cand, t))
             if log:

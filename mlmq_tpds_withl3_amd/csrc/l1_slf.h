@@ -8,9 +8,12 @@
 // threshold after a push, all of its records move to L2.
 //
 // Sizes are re-derived for 64-lane waves and many-edge pushes: the ring holds
-// kL1Cap = 512 8-byte records (4 KiB per wave); a pop takes at most 64 (one
-// per lane); the spill threshold is a launch parameter (default 64, the
-// reference's "half of the ring" rule at wave64 granularity).  One push
+// kCap 8-byte records, 512 (4 KiB per wave) unless a kernel instance asks for
+// a larger power of two (the batched kernel also comes with 1024: the LDS
+// segment size its tuner chooses); a pop takes at most 64 (one per lane);
+// the spill threshold is a launch parameter (default 64, the reference's
+// "half of the ring" rule at wave64 granularity; at most kCap - 7 x 64, so
+// that one push of a whole batch always fits).  One push
 // places all records a batch produced -- up to 64 lanes x 7 edges -- in a
 // single pass: positions come from ballots and popcounts only, so the writes
 // are issued back to back.  Head index and size are wave-uniform values in
@@ -22,7 +25,7 @@
 
 namespace mlmq {
 
-constexpr uint32_t kL1Cap = 512;
+constexpr uint32_t kL1Cap = 512;             // default ring size (records)
 constexpr uint32_t kL1Mask = kL1Cap - 1;
 constexpr uint32_t kL1Alloc = kL1Cap + 64;   // ring + one discard slot per lane
 // Block-carrying L1 (single-query kernel): ring slot i is paired with block
@@ -44,13 +47,15 @@ __device__ __forceinline__ void l1_uniform(L1State &l1) {
 }
 
 // Move up to `maxn` (<= 64) records from the front into lanes [0, k).
+template <uint32_t kCap = kL1Cap>
 __device__ __forceinline__ uint32_t l1_pop(const uint64_t *ring, L1State &l1, uint32_t maxn,
                                            uint64_t &rec) {
+    static_assert((kCap & (kCap - 1)) == 0 && kCap >= kL1Cap, "ring size");
     l1_uniform(l1);
     const uint32_t k = uniform(l1.size < maxn ? l1.size : maxn);
-    if (lane_id() < k) rec = ring[(l1.head + lane_id()) & kL1Mask];
+    if (lane_id() < k) rec = ring[(l1.head + lane_id()) & (kCap - 1)];
     wave_sync();
-    l1.head = (l1.head + k) & kL1Mask;
+    l1.head = (l1.head + k) & (kCap - 1);
     l1.size -= k;
     return k;
 }
@@ -84,11 +89,12 @@ __device__ __forceinline__ uint32_t l1_pop_hot(const uint64_t *ring, const HopBl
 
 // Spill every record of the deque to L2 (reference write-back of the whole
 // L1, core/src/l1_queue.cu:332-350).  Returns false if L2 overflowed.
+template <uint32_t kCap = kL1Cap>
 __device__ bool l1_spill_all(const uint64_t *ring, L1State &l1, const L2Queue &q, WaveStats &st) {
     bool ok = true;
     while (l1.size && ok) {
         uint64_t x = kEmpty;
-        const uint32_t k = l1_pop(ring, l1, kWave, x);
+        const uint32_t k = l1_pop<kCap>(ring, l1, kWave, x);
         ok = l2_push(q, x, lane_id() < k, st);
     }
     return ok;
@@ -97,6 +103,7 @@ __device__ bool l1_spill_all(const uint64_t *ring, L1State &l1, const L2Queue &q
 // Work donation (no reference counterpart): move up to `want` records from
 // the back of the deque -- the ones SLF would process last -- to L2, where
 // idle waves can claim them.  Returns false if L2 overflowed.
+template <uint32_t kCap = kL1Cap>
 __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, const L2Queue &q,
                           WaveStats &st) {
     l1_uniform(l1);
@@ -104,7 +111,7 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
     if (m == 0) return true;
     const uint32_t k = m < (uint32_t)kWave ? m : (uint32_t)kWave;
     uint64_t x = kEmpty;
-    if (lane_id() < k) x = ring[(l1.head + l1.size - k + lane_id()) & kL1Mask];
+    if (lane_id() < k) x = ring[(l1.head + l1.size - k + lane_id()) & (kCap - 1)];
     wave_sync();
     l1.size -= k;
     st.v[kStatDonated] += k;
@@ -113,20 +120,20 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
 
 // Push K registers of records: record k of this lane is present when bit k
 // of `bits` is set.  Requires size <= spill on entry and
-// spill + K * 64 <= kL1Cap.  With `fifo` set every record goes to the back:
+// spill + K * 64 <= kCap.  With `fifo` set every record goes to the back:
 // the reference's l1_vector_queue (core/src/l1_queue.cu:74-99), a FIFO with
 // the same spill-all rule.
 //
 // Positions come from one packed wave prefix scan (front count in the low
 // half-word, back count in the high one); every lane then writes its records
 // without further cross-lane work.  Absent records are written to the lane's
-// own slot past the ring (kL1Cap + lane), so the writes need no branches.
+// own slot past the ring (kCap + lane), so the writes need no branches.
 // Order: lane-major, then k; front records are placed downwards from the old
 // head, so the first one pushed ends next to it.
 //
 // kBlk selects what goes into the block slots of a block-carrying L1:
 // 0 = no block slots, 1 = the records' blocks `cb[k]`, 2 = kColdMark.
-template <int K, int kBlk = 0>
+template <int K, int kBlk = 0, uint32_t kCap = kL1Cap>
 __device__ __forceinline__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
                                         uint32_t bits, uint32_t spill, uint32_t fifo,
                                         const L2Queue &q, WaveStats &st, HopBlock *blk = nullptr,
@@ -163,7 +170,7 @@ __device__ __forceinline__ bool l1_push(uint64_t *ring, L1State &l1, const uint6
     for (int k = 0; k < K; ++k) {
         const bool v = (bits >> k) & 1;
         const bool f = (fbits >> k) & 1;
-        const uint32_t p = v ? ((f ? pf : pb) & kL1Mask) : kL1Cap + lane;
+        const uint32_t p = v ? ((f ? pf : pb) & (kCap - 1)) : kCap + lane;
         ring[p] = rec[k];
         if (kBlk == 1 && v) blk[p] = cb[k];
         if (kBlk == 2 && v) reinterpret_cast<uint32_t *>(blk + p)[0] = kColdMark;
@@ -171,9 +178,9 @@ __device__ __forceinline__ bool l1_push(uint64_t *ring, L1State &l1, const uint6
         pb += (v && !f) ? 1u : 0u;
     }
     wave_sync();
-    l1.head = (l1.head - total_f) & kL1Mask;
+    l1.head = (l1.head - total_f) & (kCap - 1);
     l1.size += total;
-    if (l1.size > spill) return l1_spill_all(ring, l1, q, st);
+    if (l1.size > spill) return l1_spill_all<kCap>(ring, l1, q, st);
     return true;
 }
 

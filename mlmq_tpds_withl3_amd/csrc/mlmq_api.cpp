@@ -47,7 +47,8 @@ struct mlmq_ctx {
     uint64_t slots_count = 0;
     bool slots_dirty = true;
     bool solved = false;
-    std::vector<uint32_t> ctl_host;
+    std::vector<uint32_t> ctl_host;   // control block after the last mlmq_solve
+    uint32_t last_buckets = 0, last_window = 0, last_cap_log2 = 0;
     // batch launch
     uint32_t *b_dist = nullptr;
     uint64_t b_dist_words = 0;
@@ -391,13 +392,24 @@ int check_params(const mlmq_params &p, bool batch, std::string *why) {
     if (p.l2_batch < 1 || p.l2_batch > 64) return bad(MLMQ_E_INVALID, "l2_batch must be in [1, 64]");
     if (p.min_read < 1 || p.min_read > 64) return bad(MLMQ_E_INVALID, "min_read must be in [1, 64]");
     if (p.l1_pop < 1 || p.l1_pop > 64) return bad(MLMQ_E_INVALID, "l1_pop must be in [1, 64]");
-    if (p.l1_spill < 1 || p.l1_spill > 64) return bad(MLMQ_E_INVALID, "l1_spill must be in [1, 64]");
+    if (p.l1_ring != 0 && p.l1_ring != 512 && p.l1_ring != 1024)
+        return bad(MLMQ_E_INVALID, "l1_ring must be 0, 512 or 1024");
+    if (!batch && p.l1_ring == 1024) return bad(MLMQ_E_UNSUPPORTED, "l1_ring 1024 is for batched solves");
+    {
+        // one push of a whole batch (64 lanes x 7 inline edges) must fit above the threshold
+        const int ring = p.l1_ring ? p.l1_ring : 512;
+        const int max_spill = std::min(255, ring - 448);
+        if (p.l1_spill < 1 || p.l1_spill > max_spill)
+            return bad(MLMQ_E_INVALID, ring == 512 ? "l1_spill must be in [1, 64]"
+                                                   : "l1_spill must be in [1, 255] with l1_ring 1024");
+    }
     if (p.l1_donate < 0 || p.l1_donate > 64) return bad(MLMQ_E_INVALID, "l1_donate must be in [0, 64]");
     if (batch && p.batch_waves != 4 && p.batch_waves != 8 && p.batch_waves != 16)
         return bad(MLMQ_E_INVALID, "batch_waves must be 4, 8 or 16");
     if (p.stale_check < 0 || p.stale_check > 3) return bad(MLMQ_E_INVALID, "stale_check must be 0, 1, 2 or 3");
     if (p.l1_blocks != 0 && p.l1_blocks != 1) return bad(MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return bad(MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
+    if (p.csr_stream != 0 && p.csr_stream != 1) return bad(MLMQ_E_INVALID, "csr_stream must be 0 or 1");
     if (p.light_split && (p.hub_split || p.defer_heavy))
         return bad(MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
     if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
@@ -495,7 +507,7 @@ mlmq::SolverLaunch query_launch(const mlmq_ctx *c, const QueryPlan &q, uint32_t 
                                 uint64_t *slots, uint32_t cap_log2) {
     const mlmq_params &p = q.p;
     const double timeout_s = p.timeout_s > 0 ? p.timeout_s : 60.0;
-    mlmq::SolverLaunch L;
+    mlmq::SolverLaunch L{};
     L.row = c->row;
     L.col = c->col;
     L.wgt = c->wgt;
@@ -507,6 +519,7 @@ mlmq::SolverLaunch query_launch(const mlmq_ctx *c, const QueryPlan &q, uint32_t 
     L.defer_heavy = p.defer_heavy ? 1u : 0u;
     L.strict_window = p.strict_window ? 1u : 0u;
     L.light_split = p.light_split ? 1u : 0u;
+    L.csr_stream = p.csr_stream ? 1u : 0u;
     L.hot = q.hot;
     L.dist = dist;
     L.slots = slots;
@@ -649,6 +662,8 @@ void mlmq_params_default(mlmq_params *p) {
     p->defer_heavy = 0;
     p->strict_window = 0;
     p->light_split = 0;
+    p->csr_stream = 0;
+    p->l1_ring = 512;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -838,6 +853,9 @@ int mlmq_solve(mlmq_ctx *c, int32_t src, const mlmq_params *params, mlmq_timing 
         }
         if (rc) return rc;
         c->solved = true;
+        c->last_buckets = nb;
+        c->last_window = (uint32_t)q.p.bucket_window;
+        c->last_cap_log2 = cap_log2;
         if (timing) fill_timing(c, c->ctl_host.data(), t, q.blocks, cap_log2, retries, timing);
         return MLMQ_OK;
     }
@@ -910,7 +928,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     HIP_TRY(c, hipSetDevice(c->device));
 
     int occ = 0;
-    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, &occ));
+    const int l1_cap = p.l1_ring ? p.l1_ring : 512;
+    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, l1_cap, &occ));
     if (occ < 1) return fail(c, MLMQ_E_HIP, "batch kernel cannot be resident");
     int blocks = p.work_blocks ? p.work_blocks : c->cus;
     if (blocks > occ * c->cus) blocks = occ * c->cus;
@@ -974,7 +993,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             c->b_slots_dirty = false;
         }
 
-        mlmq::SolverLaunch L;
+        mlmq::SolverLaunch L{};
         L.row = c->row;
         L.col = c->col;
         L.wgt = c->wgt;
@@ -986,6 +1005,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.defer_heavy = p.defer_heavy ? 1u : 0u;
         L.strict_window = p.strict_window ? 1u : 0u;
         L.light_split = p.light_split ? 1u : 0u;
+        L.csr_stream = p.csr_stream ? 1u : 0u;
         L.hot = false;
         L.dist = nullptr;
         L.slots = nullptr;
@@ -1017,6 +1037,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         B.qstatus = c->b_qstatus;
         B.qtime = c->b_qtime;
         B.waves = p.batch_waves;
+        B.l1_cap = l1_cap;
 
         // The launch (row reset + batch kernel) is captured once per shape and
         // replayed as a hipGraph; the source list is copied in before it.
@@ -1029,7 +1050,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             (uint64_t)(uintptr_t)c->row, (uint64_t)(uintptr_t)c->blocks,
             (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check,
             (uint64_t)L.prefetch, (uint64_t)L.max_sleep, (uint64_t)L.defer_heavy,
-            (uint64_t)L.strict_window, (uint64_t)L.light_split};
+            (uint64_t)L.strict_window, (uint64_t)L.light_split, (uint64_t)L.csr_stream,
+            (uint64_t)l1_cap};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
@@ -1127,6 +1149,24 @@ int mlmq_copy_batch_dist(mlmq_ctx *c, int32_t query, int32_t *host_out) {
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpy(host_out, c->b_dist + (uint64_t)query * (uint64_t)c->n,
                          sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToHost));
+    return MLMQ_OK;
+}
+
+int mlmq_last_queue_counts(const mlmq_ctx *c, mlmq_queue_counts *out) {
+    if (!c || !out) return MLMQ_E_INVALID;
+    if (!c->solved) return MLMQ_E_NOGRAPH;
+    std::memset(out, 0, sizeof(*out));
+    const uint32_t *h = c->ctl_host.data();
+    out->buckets = (int32_t)c->last_buckets;
+    out->window = (int32_t)c->last_window;
+    out->capacity = (int64_t)1 << c->last_cap_log2;
+    for (uint32_t b = 0; b < c->last_buckets; ++b) {
+        out->writes[b] = h[mlmq::kTailOff + b * mlmq::kStride];
+        out->reads[b] = h[mlmq::kTakenOff + b * mlmq::kStride];
+        if ((int64_t)out->writes[b] > out->capacity) ++out->wrapped;
+    }
+    out->completed = h[mlmq::kDoneOff];
+    out->epoch = h[mlmq::kBaseOff];
     return MLMQ_OK;
 }
 

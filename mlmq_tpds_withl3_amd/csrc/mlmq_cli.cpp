@@ -9,7 +9,15 @@
 //       L3_SUPPLEMENT_ORACLE (int32 .i32 file of expected distances),
 //       MLMQ_SKIP_CPU_CHECK=1 (timing-only runs: no CPU reference),
 //       MLMQ_MULTI_MODE (batch: one persistent launch for all sources, default;
-//                        sequential: one solve per source, as the reference does).
+//                        sequential: one solve per source, as the reference does),
+//       MLMQ_L2_REPORT=1 (per query, one CAPACITY_BUCKET line per bucket and a
+//                        CAPACITY_QUERY line, as the reference's diagnostic build).
+//
+// Queue audit: an L2_CAPACITY line after the first solve and an L2_FINAL line
+// per query (reads, writes and completed records of the L2 queue; the
+// reference's cu_delta_queue.cuh:165 and SSSP/sssp_run.cu:10900-10926); a
+// query whose per-bucket reads and writes, or their sum and the completed
+// count, differ exits with code 2.
 //
 // One source (default): like the reference's main, a CPU Dijkstra runs first
 // ("CPU total work count", "CPU sequential time"), then every GPU run prints
@@ -120,6 +128,39 @@ void print_row(const char *label, const int32_t *d, int32_t n) {
     std::printf("%s\n", label);
     for (int32_t i = 0; i < n && i < 30; ++i) std::printf("%d ", d[i]);
     std::printf("\n");
+}
+
+// Queue audit of the last solve (see the header comment).  False when the
+// counts do not balance.
+bool queue_audit(mlmq_ctx *ctx, bool first, bool per_bucket) {
+    mlmq_queue_counts q;
+    if (mlmq_last_queue_counts(ctx, &q) != 0) {
+        std::fprintf(stderr, "queue counts unavailable\n");
+        return false;
+    }
+    if (first)
+        std::printf("L2_CAPACITY budget=%lld record_bytes=8 buckets=%d per_bucket=%lld "
+                    "allocated_records=%lld counter_bits=32\n",
+                    (long long)(8 * q.capacity * q.buckets), q.buckets, (long long)q.capacity,
+                    (long long)(q.capacity * q.buckets));
+    long long reads = 0, writes = 0;
+    bool ok = true;
+    for (int b = 0; b < q.buckets; ++b) {
+        reads += q.reads[b];
+        writes += q.writes[b];
+        ok = ok && q.reads[b] == q.writes[b];
+        if (per_bucket)
+            std::printf("CAPACITY_BUCKET gpu=0 bucket=%d capacity=%lld writes=%u reads=%u no_wrap=%d\n", b,
+                        (long long)q.capacity, q.writes[b], q.reads[b],
+                        (long long)q.writes[b] <= q.capacity ? 1 : 0);
+    }
+    ok = ok && writes == (long long)q.completed;
+    if (per_bucket)
+        std::printf("CAPACITY_QUERY gpu=0 total=%lld completed=%u guarded=1\n", writes, q.completed);
+    std::printf("L2_FINAL gpu=0 buckets=%d reads=%lld writes=%lld completed=%u\n", q.buckets, reads, writes,
+                q.completed);
+    if (!ok) std::fprintf(stderr, "L2 conservation failed\n");
+    return ok;
 }
 
 struct Graph {
@@ -274,6 +315,7 @@ int main(int argc, char **argv) {
     const long warmups = env_long("BENCH_WARMUPS", 2, 0, 10000);
     const long repeats = env_long("BENCH_REPEATS", 3, 1, 10000);
     const bool skip_cpu = env_long("MLMQ_SKIP_CPU_CHECK", 0, 0, 1) == 1;
+    const bool report_buckets = env_long("MLMQ_L2_REPORT", 0, 0, 1) == 1;
 
     std::vector<int32_t> expect;
     if (const char *op = std::getenv("L3_SUPPLEMENT_ORACLE")) {
@@ -351,6 +393,11 @@ int main(int argc, char **argv) {
         rc = mlmq_copy_dist(ctx, dist.data());
         if (rc) {
             std::fprintf(stderr, "copy failed: %d %s\n", rc, mlmq_last_error(ctx));
+            mlmq_destroy(ctx);
+            mlmq_graph_free(hg);
+            return 2;
+        }
+        if (!queue_audit(ctx, r == 0, report_buckets)) {
             mlmq_destroy(ctx);
             mlmq_graph_free(hg);
             return 2;

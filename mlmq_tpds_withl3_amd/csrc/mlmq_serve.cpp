@@ -46,20 +46,18 @@ struct GraphEntry {
     uint64_t row_bytes = 0, ctl_bytes = 0, slot_bytes = 0;
     std::vector<Slot> free_slots;
     std::unordered_map<int32_t, uint32_t *> expected;
-    std::deque<int32_t> pending;
 };
 
 struct Lane {
     hipStream_t s = nullptr;
     hipEvent_t ev_solve = nullptr, ev_done = nullptr;
-    mlmq::ServeRequestHost *h_reqs = nullptr;   // pinned
-    mlmq::ServeRequestHost *d_reqs = nullptr;
+    uint8_t *h_reqs = nullptr;   // pinned request table (serve_request_bytes() per entry)
+    uint8_t *d_reqs = nullptr;
     bool busy = false;
-    int32_t graph = -1;
     int32_t launch = -1;
-    uint32_t cap_log2 = 0;
     std::vector<int32_t> ids;
     std::vector<Slot> slots;
+    std::vector<uint32_t> caps;   // ring capacity (log2) each request ran with
 };
 
 }  // namespace
@@ -72,6 +70,7 @@ struct mlmq_server {
     Arena arena;
     std::vector<GraphEntry> graphs;
     std::vector<Lane> lanes;
+    std::deque<int32_t> pending;   // waiting request ids, in arrival order
     hipEvent_t ev_ref = nullptr;
     std::string err;
 };
@@ -213,6 +212,41 @@ double pct(std::vector<double> v, double q) {
     return v[k - 1];
 }
 
+// Kernel settings of one request of graph g (its slot and ring capacity are
+// filled in per request).
+mlmq::SolverLaunch graph_launch(const GraphEntry &g, uint64_t timeout_ticks) {
+    mlmq::SolverLaunch L{};
+    L.row = g.v.row;
+    L.col = g.v.col;
+    L.wgt = g.v.wgt;
+    L.blocks = g.v.blocks;
+    L.hubs = g.v.hubs;
+    L.split = g.p.hub_split != 0;
+    L.stale_check = g.p.stale_check == 0 ? 2u : g.p.stale_check < 0 ? 0u : (uint32_t)g.p.stale_check;
+    L.prefetch = g.p.prefetch >= 0 ? 1u : 0u;
+    L.hot = false;
+    L.defer_heavy = 0;
+    L.strict_window = 0;
+    L.light_split = g.p.light_split ? 1u : 0u;
+    L.n = (uint32_t)g.v.n;
+    L.nbuckets = (uint32_t)g.p.buckets;
+    L.window = (uint32_t)g.p.bucket_window;
+    L.cap_log2 = g.cap_log2;
+    L.delta = g.p.delta;
+    L.batch = (uint32_t)g.p.l2_batch;
+    L.spin_limit = mlmq::kSpinLimit;
+    L.min_read = (uint32_t)g.p.min_read;
+    L.max_sleep = 16;
+    L.pop_max = (uint32_t)g.p.l1_pop;
+    L.spill = (uint32_t)g.p.l1_spill;
+    L.donate_min = 0;
+    L.timeout_ticks = timeout_ticks;
+    L.slf = true;
+    L.fifo = false;
+    L.profile = false;
+    return L;
+}
+
 int add_graph_ctx(mlmq_server *s, mlmq_ctx *ctx, const mlmq_serve_graph_params *p, int32_t *graph_id) {
     GraphEntry g;
     g.ctx = ctx;
@@ -298,7 +332,7 @@ int mlmq_server_create(int device, const mlmq_server_config *cfg, mlmq_server **
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreate(&l.ev_solve);
         if (e == hipSuccess) e = hipEventCreate(&l.ev_done);
-        const size_t tb = sizeof(mlmq::ServeRequestHost) * (size_t)c.max_batch;
+        const size_t tb = mlmq::serve_request_bytes() * (size_t)c.max_batch;
         if (e == hipSuccess) e = hipHostMalloc((void **)&l.h_reqs, tb, hipHostMallocDefault);
         if (e == hipSuccess) e = hipMalloc((void **)&l.d_reqs, tb);
     }
@@ -444,7 +478,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         r.mismatches = -1;
         r.launch = -1;
     }
-    for (GraphEntry &g : s->graphs) g.pending.clear();
+    s->pending.clear();
 
     const uint64_t timeout_ticks = (uint64_t)(s->cfg.timeout_s * (double)s->wall_khz * 1000.0);
     std::vector<int32_t> retries(nreq, 0);
@@ -454,6 +488,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
     s->arena.peak = s->arena.used;
     double wall_ms = 0.0;
     int rc = MLMQ_OK;
+    const size_t rb = mlmq::serve_request_bytes();
 
     HIP_TRY(s, hipEventRecord(s->ev_ref, s->lanes[0].s));
     HIP_TRY(s, hipEventSynchronize(s->ev_ref));
@@ -461,110 +496,90 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
     auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
 
     auto launch_lane = [&](Lane &l) -> int {
-        // Oldest waiting request decides the graph.
-        int32_t gi = -1;
-        double oldest = 0;
-        for (size_t j = 0; j < s->graphs.size(); ++j) {
-            const GraphEntry &g = s->graphs[j];
-            if (g.pending.empty()) continue;
-            const double a = reqs[g.pending.front()].arrival_s;
-            if (gi < 0 || a < oldest) {
-                gi = (int32_t)j;
-                oldest = a;
-            }
-        }
-        if (gi < 0) return 0;
-        GraphEntry &g = s->graphs[gi];
+        if (s->pending.empty()) return 0;
+        // The oldest waiting request fixes the launch's workgroup width;
+        // waiting requests of every graph with that width join it in arrival
+        // order while a slot can be found for each (the first that finds no
+        // slot ends the launch, so a large request is not overtaken forever).
+        const int32_t width = s->graphs[reqs[s->pending.front()].graph].p.waves;
         l.ids.clear();
         l.slots.clear();
-        bool any_expect = false;
-        while (!g.pending.empty() && (int32_t)l.ids.size() < s->cfg.max_batch) {
+        l.caps.clear();
+        for (auto it = s->pending.begin();
+             it != s->pending.end() && (int32_t)l.ids.size() < s->cfg.max_batch;) {
+            const int32_t id = *it;
+            const int32_t gi = reqs[id].graph;
+            if (s->graphs[gi].p.waves != width) {
+                ++it;
+                continue;
+            }
             Slot sl;
             if (!take_slot(s, gi, &sl)) break;
-            const int32_t id = g.pending.front();
-            g.pending.pop_front();
+            it = s->pending.erase(it);
             --pending;
-            mlmq::ServeRequestHost &h = l.h_reqs[l.ids.size()];
-            uint8_t *b = s->arena.base + sl.off;
-            h.src = (uint32_t)reqs[id].src;
-            h.status = 0;
-            h.fresh = sl.fresh ? 1u : 0u;
-            h.mismatches = 0;
-            h.dist = (uint32_t *)b;
-            h.ctl = (uint32_t *)(b + g.row_bytes);
-            h.slots = (uint64_t *)(b + g.row_bytes + g.ctl_bytes);
-            h.expect = nullptr;
-            if (s->cfg.verify) {
-                auto it = g.expected.find(reqs[id].src);
-                if (it != g.expected.end()) h.expect = it->second;
-            }
-            any_expect |= h.expect != nullptr;
-            h.t_start = h.t_end = 0;
             l.ids.push_back(id);
             l.slots.push_back(sl);
+            l.caps.push_back(s->graphs[gi].cap_log2);
         }
         if (l.ids.empty()) {
             if (inflight > 0) return 0;   // bytes held by running requests: wait for them
-            // Nothing runs and the slot still does not fit: it never will.
-            const int32_t id = g.pending.front();
-            g.pending.pop_front();
+            // Nothing runs and the oldest request's slot still does not fit: it never will.
+            const int32_t id = s->pending.front();
+            s->pending.pop_front();
             --pending;
             results[id].status = MLMQ_E_NOMEM;
             ++finished;
             return 1;
         }
         const uint32_t nb = (uint32_t)l.ids.size();
-        const size_t tb = sizeof(mlmq::ServeRequestHost) * nb;
-        HIP_TRY(s, hipMemcpyAsync(l.d_reqs, l.h_reqs, tb, hipMemcpyHostToDevice, l.s));
-        mlmq::SolverLaunch L{};
-        L.row = g.v.row;
-        L.col = g.v.col;
-        L.wgt = g.v.wgt;
-        L.blocks = g.v.blocks;
-        L.hubs = g.v.hubs;
-        L.split = g.p.hub_split != 0;
-        L.stale_check = g.p.stale_check == 0 ? 2u : g.p.stale_check < 0 ? 0u : (uint32_t)g.p.stale_check;
-        L.prefetch = g.p.prefetch >= 0 ? 1u : 0u;
-        L.hot = false;
-        L.defer_heavy = 0;
-        L.strict_window = 0;
-        L.light_split = g.p.light_split ? 1u : 0u;
-        L.n = (uint32_t)g.v.n;
-        L.nbuckets = (uint32_t)g.p.buckets;
-        L.window = (uint32_t)g.p.bucket_window;
-        L.cap_log2 = g.cap_log2;
-        L.delta = g.p.delta;
-        L.batch = (uint32_t)g.p.l2_batch;
-        L.spin_limit = mlmq::kSpinLimit;
-        L.min_read = (uint32_t)g.p.min_read;
-        L.max_sleep = 16;
-        L.pop_max = (uint32_t)g.p.l1_pop;
-        L.spill = (uint32_t)g.p.l1_spill;
-        L.donate_min = 0;
-        L.timeout_ticks = timeout_ticks;
         // Few requests in the launch: spread each over more waves, up to
         // spread_waves for the whole launch, as one-wave workgroups (they go
         // to distinct CUs, as in the single-query solver).
-        uint32_t wg = (uint32_t)g.p.workgroups, waves = (uint32_t)g.p.waves;
-        if (s->cfg.spread_waves > 0) {
-            const uint32_t fill = std::min<uint32_t>((uint32_t)s->cfg.spread_waves / nb, kMaxSpread);
-            if (fill > wg * waves) {
-                wg = fill;
-                waves = 1;
-            }
+        uint32_t widest = 0;
+        for (int32_t id : l.ids) {
+            const GraphEntry &g = s->graphs[reqs[id].graph];
+            widest = std::max(widest, (uint32_t)g.p.workgroups * (uint32_t)width);
         }
-        L.grid_blocks = (int)(nb * wg);
-        L.slf = true;
-        L.fifo = false;
-        L.profile = false;
-        mlmq::ServeLaunch S;
+        uint32_t fill = 0;
+        if (s->cfg.spread_waves > 0) {
+            fill = std::min<uint32_t>((uint32_t)s->cfg.spread_waves / nb, kMaxSpread);
+            if (fill <= widest) fill = 0;
+        }
+        mlmq::ServeLaunch S{};
         S.reqs = l.d_reqs;
         S.nreq = nb;
-        S.wg_per_req = wg;
-        S.waves = (int)waves;
-        HIP_TRY(s, mlmq::launch_serve(L, S, l.s));
+        S.waves = fill ? 1 : width;
+        bool any_expect = false;
+        uint32_t wg_first = 0;
+        for (uint32_t i = 0; i < nb; ++i) {
+            const int32_t id = l.ids[i];
+            const GraphEntry &g = s->graphs[reqs[id].graph];
+            const Slot &sl = l.slots[i];
+            uint8_t *b = s->arena.base + sl.off;
+            mlmq::SolverLaunch L = graph_launch(g, timeout_ticks);
+            L.cap_log2 = l.caps[i];
+            L.dist = (uint32_t *)b;
+            L.ctl = (uint32_t *)(b + g.row_bytes);
+            L.slots = (uint64_t *)(b + g.row_bytes + g.ctl_bytes);
+            const uint32_t *expect = nullptr;
+            if (s->cfg.verify) {
+                auto it = g.expected.find(reqs[id].src);
+                if (it != g.expected.end()) expect = it->second;
+            }
+            any_expect |= expect != nullptr;
+            const uint32_t wg = fill ? fill : (uint32_t)g.p.workgroups;
+            mlmq::serve_request_write(l.h_reqs, i, L, (uint32_t)reqs[id].src, sl.fresh, expect, wg_first, wg);
+            wg_first += wg;
+            S.max_n = std::max(S.max_n, L.n);
+            S.max_part_slots = std::max(S.max_part_slots, (uint64_t)L.nbuckets << L.cap_log2);
+            S.ext = S.ext || L.light_split || L.defer_heavy || L.strict_window;
+        }
+        S.grid = wg_first;
+        const size_t tb = rb * nb;
+        HIP_TRY(s, hipMemcpyAsync(l.d_reqs, l.h_reqs, tb, hipMemcpyHostToDevice, l.s));
+        HIP_TRY(s, mlmq::launch_serve(S, l.s));
         HIP_TRY(s, hipEventRecord(l.ev_solve, l.s));
-        if (any_expect) HIP_TRY(s, mlmq::launch_serve_verify(S, L.n, l.s));
+        if (any_expect) HIP_TRY(s, mlmq::launch_serve_verify(S, l.s));
         HIP_TRY(s, hipMemcpyAsync(l.h_reqs, l.d_reqs, tb, hipMemcpyDeviceToHost, l.s));
         HIP_TRY(s, hipEventRecord(l.ev_done, l.s));
         const double t = now_ms();
@@ -574,9 +589,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
             results[id].batch_size = (int32_t)nb;
         }
         l.busy = true;
-        l.graph = gi;
         l.launch = launches++;
-        l.cap_log2 = g.cap_log2;
         inflight += nb;
         peak_inflight = std::max(peak_inflight, inflight);
         batch_sum += nb;
@@ -586,47 +599,55 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
     auto complete_lane = [&](Lane &l) -> int {
         float done_ms = 0.0f;
         HIP_TRY(s, hipEventElapsedTime(&done_ms, s->ev_ref, l.ev_solve));
-        GraphEntry &g = s->graphs[l.graph];
+        std::vector<int32_t> rerun;   // overflowed requests, in launch order
         for (size_t i = 0; i < l.ids.size(); ++i) {
             const int32_t id = l.ids[i];
-            const mlmq::ServeRequestHost &h = l.h_reqs[i];
+            const int32_t gi = reqs[id].graph;
+            GraphEntry &g = s->graphs[gi];
+            uint32_t status = 0, mism = 0;
+            uint64_t ts = 0, te = 0;
+            mlmq::serve_request_read(l.h_reqs, (uint32_t)i, &status, &mism, &ts, &te);
             Slot sl = l.slots[i];
             mlmq_serve_result &r = results[id];
-            const int st = device_status(h.status);
+            const int st = device_status(status);
             if (st == MLMQ_E_QUEUE_OVERFLOW && g.cap_log2 < 30) {
                 // Grow the graph's partitions once per launch and run again.
-                if (g.cap_log2 == l.cap_log2) {
+                if (g.cap_log2 == l.caps[i]) {
                     drop_free_slots(s, g);
                     ++g.cap_log2;
                     size_slots(g);
                 }
                 sl.fresh = true;
-                give_back(s, l.graph, sl);
+                give_back(s, gi, sl);
                 ++retries[id];
-                g.pending.push_front(id);
-                ++pending;
+                rerun.push_back(id);
                 continue;
             }
             r.status = st;
             r.retries = retries[id];
             r.latency_ms = (double)done_ms - 1000.0 * reqs[id].arrival_s;
-            r.device_ms = h.t_end > h.t_start ? (double)(h.t_end - h.t_start) / (double)s->wall_khz : 0.0;
-            r.mismatches = h.expect ? (int32_t)h.mismatches : -1;
+            r.device_ms = te > ts ? (double)(te - ts) / (double)s->wall_khz : 0.0;
+            const bool verified = s->cfg.verify && g.expected.count(reqs[id].src) != 0;
+            r.mismatches = verified ? (int32_t)mism : -1;
             if (st == MLMQ_OK && reqs[id].dist_out) {
-                HIP_TRY(s, hipMemcpyAsync(reqs[id].dist_out, h.dist, sizeof(int32_t) * (size_t)g.v.n,
-                                          hipMemcpyDeviceToHost, l.s));
+                HIP_TRY(s, hipMemcpyAsync(reqs[id].dist_out, s->arena.base + sl.off,
+                                          sizeof(int32_t) * (size_t)g.v.n, hipMemcpyDeviceToHost, l.s));
                 HIP_TRY(s, hipStreamSynchronize(l.s));
             }
             // A failed solve leaves records in its partition: refill before reuse.
-            if (st != MLMQ_OK) sl.fresh = true;
-            else sl.fresh = false;
-            give_back(s, l.graph, sl);
+            sl.fresh = st != MLMQ_OK;
+            give_back(s, gi, sl);
             ++finished;
+        }
+        for (auto it = rerun.rbegin(); it != rerun.rend(); ++it) {
+            s->pending.push_front(*it);
+            ++pending;
         }
         inflight -= l.ids.size();
         l.busy = false;
         l.ids.clear();
         l.slots.clear();
+        l.caps.clear();
         wall_ms = std::max(wall_ms, (double)done_ms);
         return MLMQ_OK;
     };
@@ -639,7 +660,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
             if (s->cfg.max_pending > 0 && pending >= s->cfg.max_pending) {
                 ++rejected;   // result already says MLMQ_E_REJECTED
             } else {
-                s->graphs[reqs[next].graph].pending.push_back(next);
+                s->pending.push_back(next);
                 ++pending;
             }
             ++next;
@@ -691,7 +712,7 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
             for (size_t i = 0; i < l.slots.size(); ++i) {
                 Slot sl = l.slots[i];
                 sl.fresh = true;
-                give_back(s, l.graph, sl);
+                give_back(s, reqs[l.ids[i]].graph, sl);
             }
             l.busy = false;
         }

@@ -71,6 +71,7 @@ struct SolverLaunch {
     uint32_t defer_heavy;   // 1: defer heavy records beyond the head bucket
     uint32_t strict_window; // 1: window advance waits for all taken records to be credited
     uint32_t light_split;   // 1: light/heavy edge split (sorted adjacency, light counts set)
+    uint32_t csr_stream;    // 1: non-temporal loads of CSR columns and weights
     bool profile;   // per-phase clock build of the kernel
 };
 
@@ -84,34 +85,34 @@ struct BatchLaunch {
     uint32_t *qstatus;
     uint64_t *qtime;
     int waves;   // waves per workgroup (= per query): 4, 8 or 16
+    int l1_cap;  // L1 ring records per wave: 512 or 1024
 };
 
-// Host mirror of the device-side ServeRequest (sssp_kernels.hip); the layout
-// must match field for field.
-struct ServeRequestHost {
-    uint32_t src;
-    uint32_t status;
-    uint32_t fresh;
-    uint32_t mismatches;
-    uint32_t *dist;
-    uint32_t *ctl;
-    uint64_t *slots;
-    const uint32_t *expect;
-    uint64_t t_start;
-    uint64_t t_end;
-};
-static_assert(sizeof(ServeRequestHost) == 64, "request layout");
+// Serving launches: a table of requests, each with its own graph, queue
+// partition and settings (the device layout is private to sssp_kernels.hip;
+// the host fills and reads a table through these functions).
+size_t serve_request_bytes();
+// Request i of a host-side table: L names the graph, the slot (L.dist,
+// L.ctl, L.slots) and the settings; workgroups [wg_first, wg_first +
+// wg_count) of the launch serve it.
+void serve_request_write(void *table, uint32_t i, const SolverLaunch &L, uint32_t src, bool fresh,
+                         const uint32_t *expect, uint32_t wg_first, uint32_t wg_count);
+void serve_request_read(const void *table, uint32_t i, uint32_t *status, uint32_t *mismatches,
+                        uint64_t *t_start, uint64_t *t_end);
 
 struct ServeLaunch {
-    ServeRequestHost *reqs;   // device table, nreq entries
+    void *reqs;               // device table, nreq entries, wg_first ascending from 0
     uint32_t nreq;
-    uint32_t wg_per_req;      // workgroups sharing one request's partition
-    int waves;                // waves per workgroup: 4, 8 or 16
+    uint32_t grid;            // workgroups in all (sum of wg_count)
+    uint32_t max_n;           // longest distance row
+    uint64_t max_part_slots;  // largest queue partition (ring slots)
+    int waves;                // waves per workgroup: 1, 4, 8 or 16
+    bool ext;                 // some request needs light_split / defer_heavy / strict_window
 };
 
-hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t s);
-hipError_t launch_serve_verify(const ServeLaunch &S, uint32_t n, hipStream_t s);
-hipError_t batch_occupancy(int waves, int *blocks_per_cu);
+hipError_t launch_serve(const ServeLaunch &S, hipStream_t s);
+hipError_t launch_serve_verify(const ServeLaunch &S, hipStream_t s);
+hipError_t batch_occupancy(int waves, int l1_cap, int *blocks_per_cu);
 hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
                               uint32_t *gctl, hipStream_t s);
 hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s);

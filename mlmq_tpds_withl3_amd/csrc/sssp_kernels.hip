@@ -36,7 +36,7 @@ struct SolveArgs {
     // persistent loop: lims = min_read | pop_max << 8 | spill << 16 |
     // max_sleep << 24 (each <= 64); flags = fifo | split << 1 |
     // stale_check << 2 | prefetch << 4 | defer_heavy << 5 | light_split << 6 |
-    // donate_min << 8.
+    // donate_min << 8 | csr_stream << 16.
     uint32_t lims;
     uint32_t flags;
     __device__ __forceinline__ uint32_t min_read() const { return lims & 0xFFu; }
@@ -50,6 +50,7 @@ struct SolveArgs {
     __device__ __forceinline__ bool defer_heavy() const { return (flags >> 5) & 1u; }
     __device__ __forceinline__ bool light_split() const { return (flags >> 6) & 1u; }
     __device__ __forceinline__ uint32_t donate_min() const { return (flags >> 8) & 0xFFu; }
+    __device__ __forceinline__ bool csr_stream() const { return (flags >> 16) & 1u; }
     uint64_t timeout_ticks;
 };
 
@@ -67,6 +68,14 @@ struct PhaseClock {
         }
     }
 };
+
+// A CSR column or weight of a heavy vertex.  With csr_stream the load is
+// non-temporal: an edge list is read once per expansion, and keeping it out
+// of the XCD's L2 leaves room for the distance lines its pre-checks and
+// atomics hit.
+__device__ __forceinline__ uint32_t ld_csr(const uint32_t *p, bool stream) {
+    return stream ? __builtin_nontemporal_load(p) : *p;
+}
 
 __device__ __forceinline__ uint32_t word_of(const HopBlock &b, int w) {
     const uint4 &q = b.q[w >> 2];
@@ -99,7 +108,7 @@ static_assert(kHeavyUnroll * 64 < 1024, "relax_slots packs per-wave sums in 10-b
 // ids on every lane, active or not) and tentative distances nd[] are loaded;
 // push the improved destinations.  `mid()` runs after the pre-check loads are
 // issued and before they are waited for; any loads it issues stay in flight.
-template <bool kSLF, int kBlk, typename Mid>
+template <bool kSLF, int kBlk, uint32_t kCap, typename Mid>
 __device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act)[kHeavyUnroll],
                                             const uint32_t (&c)[kHeavyUnroll],
                                             const uint32_t (&nd)[kHeavyUnroll], uint64_t *ring,
@@ -148,7 +157,7 @@ __device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act
     bool ok = true;
     if (sums >> 20) {
         if (kSLF) {
-            ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0>(ring, l1, out, bits, a.spill(), a.fifo(),
+            ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(),
                                                      a.q, st, blk);
         } else {
 #pragma unroll
@@ -159,7 +168,7 @@ __device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act
     return ok;
 }
 
-template <bool kSLF, int kBlk>
+template <bool kSLF, int kBlk, uint32_t kCap = kL1Cap>
 __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
                             uint32_t hdeg, uint64_t *ring, HopBlock *blk, L1State &l1,
                             uint32_t *map, WaveStats &st) {
@@ -185,8 +194,8 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
             const uint32_t e =
                 act[j] ? read_lane(hstart, ow) + (pj - read_lane(pex, ow)) * kWave + lane : 0u;
             o_d[j] = read_lane(d, ow);
-            c[j] = a.col[e];
-            w[j] = a.wgt[e];
+            c[j] = ld_csr(a.col + e, a.csr_stream());
+            w[j] = ld_csr(a.wgt + e, a.csr_stream());
         }
     };
     if (ptotal) {
@@ -202,7 +211,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
             const uint32_t nb = pb + kHeavyUnroll;
             bool act2[kHeavyUnroll];
             uint32_t c2[kHeavyUnroll], w2[kHeavyUnroll], o_d2[kHeavyUnroll];
-            ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st,
+            ok = relax_slots<kSLF, kBlk, kCap>(a, act, c, nd, ring, blk, l1, st,
                                          [&]() { load_pieces(nb, act2, c2, w2, o_d2); });
             if (!ok || nb >= ptotal) break;
 #pragma unroll
@@ -247,15 +256,15 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
             nd[j] = 0;
             if (act[j]) {
                 const uint32_t e = o_off + base + j * kWave + lane;
-                c[j] = a.col[e];
-                nd[j] = a.wgt[e];
+                c[j] = ld_csr(a.col + e, a.csr_stream());
+                nd[j] = ld_csr(a.wgt + e, a.csr_stream());
             }
         }
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
-        ok = relax_slots<kSLF, kBlk>(a, act, c, nd, ring, blk, l1, st, []() {});
+        ok = relax_slots<kSLF, kBlk, kCap>(a, act, c, nd, ring, blk, l1, st, []() {});
     }
     return ok;
 }
@@ -302,7 +311,7 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 // which keep the whole device busy, are bound by the memory-side atomic
 // traffic and gain from it; the single-query kernel, bound by the latency
 // of its critical path, does not.
-template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre>
+template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre, uint32_t kCap = kL1Cap>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
                               const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
                               uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &sunk,
@@ -504,7 +513,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     bool ok = true;
     if (sums >> 16) {
         if (kSLF) {
-            ok = l1_push<kInlineEdges, kHot ? 1 : 0>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
+            ok = l1_push<kInlineEdges, kHot ? 1 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
                                                      st, blk, cb);
         } else {
 #pragma unroll
@@ -514,7 +523,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         clk.lap(st, kClkFlush);
     }
     if (ok && ballot(heavy)) {
-        ok = relax_heavy<kSLF, kHot ? 1 : 0>(a, heavy, d, hstart, deg, ring, blk, l1, map, st);
+        ok = relax_heavy<kSLF, kHot ? 1 : 0, kCap>(a, heavy, d, hstart, deg, ring, blk, l1, map, st);
         clk.lap(st, kClkEdges);
     }
     // The heavy edges wait for a heavy-phase record one bucket later: by then
@@ -528,7 +537,8 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 // loop, SS/SSSP/sssp_run.cu:507-584).  Returns when the query's exit flag is
 // seen, after an error, or at the deadline.  `leader` waves also run the
 // termination check while idle.
-template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false, bool kPre = false>
+template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false, bool kPre = false,
+          uint32_t kCap = kL1Cap>
 __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
                            uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
                            uint64_t deadline, HopBlock *blk = nullptr) {
@@ -566,7 +576,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
             nin = l1_pop_hot(ring, blk, l1, a.pop_max(), rec, hot, hw);
             clk.lap(st, kClkL1Pop);
         } else if (kSLF) {
-            nin = l1_pop(ring, l1, a.pop_max(), rec);
+            nin = l1_pop<kCap>(ring, l1, a.pop_max(), rec);
             clk.lap(st, kClkL1Pop);
         }
         // Top up from L2 below min_read.  A block-carrying L1 may hold back
@@ -610,12 +620,12 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf, kHot, kExt, kPre>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
+        if (!process_batch<kSLF, kProf, kHot, kExt, kPre, kCap>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
                                               st, clk))
             break;
         if (kSLF && hungry && l1.size > a.donate_min()) {
             // (uniform: hungry is a scalar load result)
-            if (!l1_donate(ring, l1, uniform(l1.size) / 2, q, st)) break;
+            if (!l1_donate<kCap>(ring, l1, uniform(l1.size) / 2, q, st)) break;
             clk.lap(st, kClkFlush);
         }
         if ((st.v[kStatBatches] & 31) == 0) {
@@ -694,9 +704,9 @@ struct BatchArgs {
     uint64_t *qtime;           // per query: wall-clock ticks at start and end
 };
 
-template <int kWaves, bool kProf, bool kExt = false>
+template <int kWaves, bool kProf, bool kExt = false, uint32_t kCap = kL1Cap>
 __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
-    __shared__ uint64_t s_ring[kWaves][kL1Alloc];
+    __shared__ uint64_t s_ring[kWaves][kCap + 64];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
     __shared__ uint32_t s_query;
@@ -730,8 +740,8 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
         __threadfence();
         __syncthreads();
         a.dist = b.dist_all + (uint64_t)qi * a.n;
-        run_worker<true, kProf, false, kExt, true>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st, clk,
-                                             deadline);
+        run_worker<true, kProf, false, kExt, true, kCap>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st,
+                                                   clk, deadline);
         __syncthreads();
         if (threadIdx.x == 0) {
             b.qtime[2 * qi + 1] = wall_clock64();
@@ -753,57 +763,67 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
     flush_stats(b.gctl, st);
 }
 
-// ---- serving: requests of one resident graph, several workgroups each ----
+// ---- serving: requests of any resident graphs, several workgroups each ----
 //
-// A serving launch carries a table of requests for one graph.  Each request
-// owns a slot handed out by the host's admission control: a distance row and
-// a queue partition (control block + buckets).  serve_reset_kernel prepares
-// the slots (row fill, control reset, empty-slot fill of fresh partitions,
-// seed record); sssp_serve_kernel gives every request `wg_per_req`
-// workgroups that share its partition; serve_verify_kernel compares finished
-// rows with expected rows when the caller registered them.
+// A serving launch carries a table of requests.  Each request names its own
+// graph and settings (a full SolveArgs: CSR, hop blocks, queue parameters)
+// and owns a slot handed out by the host's admission control: a distance row
+// and a queue partition (control block + buckets).  Requests of different
+// graphs and sizes share one launch.  serve_reset_kernel prepares the slots
+// (row fill, control reset, empty-slot fill of fresh partitions, seed
+// record); sssp_serve_kernel gives every request its `wg_count` workgroups,
+// which share its partition; serve_verify_kernel compares finished rows with
+// expected rows when the caller registered them.
 struct ServeRequest {
+    SolveArgs a;              // graph, partition (dist, q.ctl, q.slots) and settings
     uint32_t src;
     uint32_t status;          // out: device error code
     uint32_t fresh;           // partition content unknown: fill with empty slots
     uint32_t mismatches;      // out (verification)
-    uint32_t *dist;
-    uint32_t *ctl;
-    uint64_t *slots;
+    uint32_t wg_first;        // first workgroup of this request in the launch
+    uint32_t wg_count;        // workgroups sharing its partition
     const uint32_t *expect;   // optional expected row
     uint64_t t_start;         // out: wall-clock ticks of the first workgroup's start
     uint64_t t_end;           // out: latest workgroup end
 };
 
 // grid (x: row chunks, y: requests)
-__global__ __launch_bounds__(256) void serve_reset_kernel(ServeRequest *reqs, uint32_t n,
-                                                          uint64_t part_slots) {
-    const ServeRequest r = reqs[blockIdx.y];
+__global__ __launch_bounds__(256) void serve_reset_kernel(ServeRequest *reqs) {
+    const ServeRequest &r = reqs[blockIdx.y];
+    const uint32_t n = r.a.n;
+    const uint64_t part_slots = (uint64_t)r.a.q.nbuckets << r.a.q.cap_log2;
+    const uint32_t src = r.src;
+    uint32_t *dist = r.a.dist;
+    uint32_t *ctl = r.a.q.ctl;
+    uint64_t *slots = r.a.q.slots;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint64_t i = t0; i < n; i += stride) r.dist[i] = i == r.src ? 0u : kUnreached;
+    for (uint64_t i = t0; i < n; i += stride) dist[i] = i == src ? 0u : kUnreached;
     if (r.fresh)
-        for (uint64_t i = t0 + 1; i < part_slots; i += stride) r.slots[i] = kEmpty;
+        for (uint64_t i = t0 + 1; i < part_slots; i += stride) slots[i] = kEmpty;
     if (blockIdx.x == 0) {
-        for (uint32_t i = threadIdx.x; i < kStatsOff; i += blockDim.x) r.ctl[i] = i == kTailOff ? 1u : 0u;
-        if (threadIdx.x == 0) r.slots[0] = make_rec(r.src, 0u);
+        for (uint32_t i = threadIdx.x; i < kStatsOff; i += blockDim.x) ctl[i] = i == kTailOff ? 1u : 0u;
+        if (threadIdx.x == 0) slots[0] = make_rec(src, 0u);
     }
 }
 
 template <int kWaves, bool kExt = false>
-__global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base, ServeRequest *reqs,
-                                                                 uint32_t wg_per_req) {
+__global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(ServeRequest *reqs, uint32_t nreq) {
     __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
     __shared__ uint64_t s_stage[kWaves][kWave];
 
     const uint32_t w = threadIdx.x / kWave;
-    const uint32_t ri = blockIdx.x / wg_per_req, part = blockIdx.x % wg_per_req;
-    ServeRequest *r = reqs + ri;
-    SolveArgs a = base;
-    a.dist = r->dist;
-    a.q.ctl = r->ctl;
-    a.q.slots = r->slots;
+    // this workgroup's request: the last one whose first workgroup is <= ours
+    uint32_t lo = 0, hi = nreq;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (reqs[mid].wg_first <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    ServeRequest *r = reqs + lo;
+    const uint32_t part = blockIdx.x - r->wg_first;
+    const SolveArgs a = r->a;
     if (part == 0 && threadIdx.x == 0) r->t_start = wall_clock64();
 
     WaveStats st;
@@ -835,11 +855,12 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(SolveArgs base,
 }
 
 // grid (x: row chunks, y: requests)
-__global__ __launch_bounds__(256) void serve_verify_kernel(ServeRequest *reqs, uint32_t n) {
+__global__ __launch_bounds__(256) void serve_verify_kernel(ServeRequest *reqs) {
     ServeRequest *r = reqs + blockIdx.y;
     const uint32_t *expect = r->expect;
     if (!expect) return;
-    const uint32_t *dist = r->dist;
+    const uint32_t *dist = r->a.dist;
+    const uint32_t n = r->a.n;
     uint32_t bad = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -1018,12 +1039,13 @@ hipError_t solver_occupancy(bool slf, bool hot, int *blocks_per_cu) {
 }
 
 template <int kW>
-static const void *batch_fn() {
-    return reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false>);
+static const void *batch_fn(int cap) {
+    return cap == 1024 ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, false, 1024>)
+                       : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, false, kL1Cap>);
 }
 
-hipError_t batch_occupancy(int waves, int *blocks_per_cu) {
-    const void *fn = waves == 16 ? batch_fn<16>() : waves == 8 ? batch_fn<8>() : batch_fn<4>();
+hipError_t batch_occupancy(int waves, int l1_cap, int *blocks_per_cu) {
+    const void *fn = waves == 16 ? batch_fn<16>(l1_cap) : waves == 8 ? batch_fn<8>(l1_cap) : batch_fn<4>(l1_cap);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
 }
 
@@ -1050,7 +1072,7 @@ static SolveArgs make_args(const SolverLaunch &L) {
              (L.max_sleep & 0xFFu) << 24;
     a.flags = (L.fifo ? 1u : 0u) | (L.hubs && L.split ? 2u : 0u) | (L.stale_check & 3u) << 2 |
               (L.prefetch ? 16u : 0u) | (L.defer_heavy ? 32u : 0u) | (L.light_split ? 64u : 0u) |
-              (L.donate_min & 0xFFu) << 8;
+              (L.donate_min & 0xFFu) << 8 | (L.csr_stream ? 1u << 16 : 0u);
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }
@@ -1061,6 +1083,23 @@ hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const
     hipLaunchKernelGGL(batch_reset_kernel, dim3(grid_for(work, 256)), dim3(256), 0, s, dist_all, n,
                        nq, sources, gctl);
     return hipGetLastError();
+}
+
+template <int kW, bool kExt, uint32_t kCap>
+static void launch_batch_instance(const dim3 &grid, const BatchArgs &b, hipStream_t s) {
+    hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap>), grid, dim3(kW * kWave), 0, s, b);
+}
+
+template <int kW, bool kExt>
+static void launch_batch_cap(const dim3 &grid, const BatchArgs &b, int cap, hipStream_t s) {
+    if (cap == 1024) launch_batch_instance<kW, kExt, 1024>(grid, b, s);
+    else launch_batch_instance<kW, kExt, kL1Cap>(grid, b, s);
+}
+
+template <int kW>
+static void launch_batch_ext(const dim3 &grid, const BatchArgs &b, bool ext, int cap, hipStream_t s) {
+    if (ext) launch_batch_cap<kW, true>(grid, b, cap, s);
+    else launch_batch_cap<kW, false>(grid, b, cap, s);
 }
 
 hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s) {
@@ -1076,57 +1115,71 @@ hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t
     b.qtime = B.qtime;
     const dim3 grid(L.grid_blocks);
     const bool ext = L.light_split || L.defer_heavy || L.strict_window;
-    if (B.waves == 16 && ext)
-        hipLaunchKernelGGL((sssp_batch_kernel<16, false, true>), grid, dim3(16 * kWave), 0, s, b);
-    else if (B.waves == 16)
-        hipLaunchKernelGGL((sssp_batch_kernel<16, false>), grid, dim3(16 * kWave), 0, s, b);
-    else if (B.waves == 8 && ext)
-        hipLaunchKernelGGL((sssp_batch_kernel<8, false, true>), grid, dim3(8 * kWave), 0, s, b);
-    else if (B.waves == 8)
-        hipLaunchKernelGGL((sssp_batch_kernel<8, false>), grid, dim3(8 * kWave), 0, s, b);
-    else if (ext)
-        hipLaunchKernelGGL((sssp_batch_kernel<4, false, true>), grid, dim3(4 * kWave), 0, s, b);
-    else
-        hipLaunchKernelGGL((sssp_batch_kernel<4, false>), grid, dim3(4 * kWave), 0, s, b);
+    if (B.waves == 16) launch_batch_ext<16>(grid, b, ext, B.l1_cap, s);
+    else if (B.waves == 8) launch_batch_ext<8>(grid, b, ext, B.l1_cap, s);
+    else launch_batch_ext<4>(grid, b, ext, B.l1_cap, s);
     return hipGetLastError();
 }
 
-hipError_t launch_serve(const SolverLaunch &L, const ServeLaunch &S, hipStream_t s) {
+size_t serve_request_bytes() { return sizeof(ServeRequest); }
+
+void serve_request_write(void *table, uint32_t i, const SolverLaunch &L, uint32_t src, bool fresh,
+                         const uint32_t *expect, uint32_t wg_first, uint32_t wg_count) {
+    ServeRequest &r = static_cast<ServeRequest *>(table)[i];
+    r.a = make_args(L);
+    r.src = src;
+    r.status = 0;
+    r.fresh = fresh ? 1u : 0u;
+    r.mismatches = 0;
+    r.wg_first = wg_first;
+    r.wg_count = wg_count;
+    r.expect = expect;
+    r.t_start = r.t_end = 0;
+}
+
+void serve_request_read(const void *table, uint32_t i, uint32_t *status, uint32_t *mismatches,
+                        uint64_t *t_start, uint64_t *t_end) {
+    const ServeRequest &r = static_cast<const ServeRequest *>(table)[i];
+    *status = r.status;
+    *mismatches = r.mismatches;
+    *t_start = r.t_start;
+    *t_end = r.t_end;
+}
+
+hipError_t launch_serve(const ServeLaunch &S, hipStream_t s) {
     if (S.nreq == 0) return hipSuccess;
-    SolveArgs a = make_args(L);
-    auto *reqs = reinterpret_cast<ServeRequest *>(S.reqs);
-    const uint64_t part_slots = (uint64_t)L.nbuckets << L.cap_log2;
-    const uint64_t longest = part_slots > L.n ? part_slots : (uint64_t)L.n;
+    auto *reqs = static_cast<ServeRequest *>(S.reqs);
+    const uint64_t longest = S.max_part_slots > S.max_n ? S.max_part_slots : (uint64_t)S.max_n;
     uint32_t chunks = (uint32_t)((longest + 256 * 64 - 1) / (256 * 64));
     if (chunks > 512) chunks = 512;
-    hipLaunchKernelGGL(serve_reset_kernel, dim3(chunks, S.nreq), dim3(256), 0, s, reqs, L.n, part_slots);
-    const dim3 grid(S.nreq * S.wg_per_req);
-    const bool ext = L.light_split || L.defer_heavy || L.strict_window;
-    if (S.waves == 16 && ext)
-        hipLaunchKernelGGL((sssp_serve_kernel<16, true>), grid, dim3(16 * kWave), 0, s, a, reqs, S.wg_per_req);
+    hipLaunchKernelGGL(serve_reset_kernel, dim3(chunks, S.nreq), dim3(256), 0, s, reqs);
+    const dim3 grid(S.grid);
+    const uint32_t nr = S.nreq;
+    if (S.waves == 16 && S.ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<16, true>), grid, dim3(16 * kWave), 0, s, reqs, nr);
     else if (S.waves == 16)
-        hipLaunchKernelGGL((sssp_serve_kernel<16>), grid, dim3(16 * kWave), 0, s, a, reqs, S.wg_per_req);
-    else if (S.waves == 8 && ext)
-        hipLaunchKernelGGL((sssp_serve_kernel<8, true>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
+        hipLaunchKernelGGL((sssp_serve_kernel<16>), grid, dim3(16 * kWave), 0, s, reqs, nr);
+    else if (S.waves == 8 && S.ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<8, true>), grid, dim3(8 * kWave), 0, s, reqs, nr);
     else if (S.waves == 8)
-        hipLaunchKernelGGL((sssp_serve_kernel<8>), grid, dim3(8 * kWave), 0, s, a, reqs, S.wg_per_req);
-    else if (S.waves == 4 && ext)
-        hipLaunchKernelGGL((sssp_serve_kernel<4, true>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
+        hipLaunchKernelGGL((sssp_serve_kernel<8>), grid, dim3(8 * kWave), 0, s, reqs, nr);
+    else if (S.waves == 4 && S.ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<4, true>), grid, dim3(4 * kWave), 0, s, reqs, nr);
     else if (S.waves == 4)
-        hipLaunchKernelGGL((sssp_serve_kernel<4>), grid, dim3(4 * kWave), 0, s, a, reqs, S.wg_per_req);
-    else if (ext)
-        hipLaunchKernelGGL((sssp_serve_kernel<1, true>), grid, dim3(kWave), 0, s, a, reqs, S.wg_per_req);
+        hipLaunchKernelGGL((sssp_serve_kernel<4>), grid, dim3(4 * kWave), 0, s, reqs, nr);
+    else if (S.ext)
+        hipLaunchKernelGGL((sssp_serve_kernel<1, true>), grid, dim3(kWave), 0, s, reqs, nr);
     else
-        hipLaunchKernelGGL((sssp_serve_kernel<1>), grid, dim3(kWave), 0, s, a, reqs, S.wg_per_req);
+        hipLaunchKernelGGL((sssp_serve_kernel<1>), grid, dim3(kWave), 0, s, reqs, nr);
     return hipGetLastError();
 }
 
-hipError_t launch_serve_verify(const ServeLaunch &S, uint32_t n, hipStream_t s) {
+hipError_t launch_serve_verify(const ServeLaunch &S, hipStream_t s) {
     if (S.nreq == 0) return hipSuccess;
-    uint32_t chunks = (uint32_t)(((uint64_t)n + 256 * 16 - 1) / (256 * 16));
+    uint32_t chunks = (uint32_t)(((uint64_t)S.max_n + 256 * 16 - 1) / (256 * 16));
     if (chunks > 256) chunks = 256;
     hipLaunchKernelGGL(serve_verify_kernel, dim3(chunks, S.nreq), dim3(256), 0, s,
-                       reinterpret_cast<ServeRequest *>(S.reqs), n);
+                       static_cast<ServeRequest *>(S.reqs));
     return hipGetLastError();
 }
 

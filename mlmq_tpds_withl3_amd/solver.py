@@ -19,7 +19,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import BatchTiming, GraphInfo, MLMQError, Params, Timing, check  # noqa: F401
+from ._lib import BatchTiming, GraphInfo, MLMQError, Params, QueueCounts, Timing, check  # noqa: F401
 
 UNREACHED = 0x7FFFFFFF
 
@@ -139,6 +139,13 @@ class Solver:
         out = np.empty(self.n, dtype=np.int32)
         check(self._L.mlmq_copy_dist(self._h, out.ctypes.data), self._h)
         return out
+
+    def queue_counts(self) -> dict:
+        """Per-bucket reservations and reads of the last solve, the credits
+        returned, the window advances and the ring capacity."""
+        q = QueueCounts()
+        check(self._L.mlmq_last_queue_counts(self._h, ctypes.byref(q)), self._h)
+        return q.as_dict()
 
     def dist_device_ptr(self) -> int:
         p = ctypes.c_void_p()

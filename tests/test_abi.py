@@ -84,10 +84,12 @@ def test_ctypes_layouts_match_c(tmp_path):
     structs = {
         "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph", "hub_split",
                                        "stale_check", "prefetch", "l1_blocks", "idle_backoff", "defer_heavy",
-                                       "strict_window", "light_split", "reserved"]),
+                                       "strict_window", "light_split", "csr_stream", "l1_ring", "reserved"]),
         "mlmq_timing": (_lib.Timing, ["donated", "hub_chunks"]),
         "mlmq_batch_timing": (_lib.BatchTiming, None),
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
+        "mlmq_queue_counts": (_lib.QueueCounts, ["capacity", "writes", "reads", "completed", "wrapped",
+                                                 "reserved"]),
         "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "spread_waves", "reserved"]),
         "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "light_split",
                                                               "prefetch", "stale_check", "reserved"]),
@@ -169,3 +171,24 @@ def test_serve_params_min_read_default_and_range():
     assert L.mlmq_serve_graph_params_resolve(ctypes.byref(g), 1000, 4000, ctypes.byref(out)) == 0
     assert (out.delta, out.buckets, out.bucket_window, out.l2_batch, out.min_read, out.l1_pop,
             out.waves, out.workgroups) == (200000, 16, 4, 8, 8, 32, 8, 1)
+
+
+def test_l1_ring_ranges():
+    """The L1 ring size and the spill threshold it allows (one batch's pushes,
+    64 lanes x 7 edges, must fit above the threshold), checked without a
+    device."""
+    from mlmq_tpds_withl3_amd.solver import default_params
+    L = _lib.load()
+
+    def rc(batch, **kw):
+        return L.mlmq_params_check(ctypes.byref(default_params(**kw)), batch)
+
+    assert default_params().l1_ring == 512
+    assert rc(1, l1_ring=1024, l1_spill=255) == 0
+    assert rc(1, l1_ring=1024, l1_spill=256) == -1
+    assert rc(1, l1_ring=512, l1_spill=64) == 0
+    assert rc(1, l1_ring=512, l1_spill=65) == -1
+    assert rc(1, l1_ring=0, l1_spill=64) == 0
+    assert rc(1, l1_ring=768) == -1
+    assert rc(0, l1_ring=1024) == -9          # single solves keep the 512-record ring
+    assert rc(0, csr_stream=2) == -1

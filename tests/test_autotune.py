@@ -161,3 +161,37 @@ def test_tune_single_confirms_against_lucky_draws():
     grid = [dict(l1_pop=64), dict(l1_pop=32)]
     r = A.tune_single(s, 0, (100,), grid=grid, repeats=1)
     assert r["params"]["l1_pop"] == 32 and r["device_ms"] == 5.0
+
+
+def test_tune_single_averages_over_tuning_sources():
+    # source 1 prefers delta 50, source 2 prefers delta 200; both together
+    # favour delta 100 (mean of the two costs)
+    seen_src = []
+
+    class S(FakeSolver):
+        def solve(self, src, p):
+            seen_src.append(src)
+            d = self.params_dict(p)
+            best = {1: 50, 2: 200}[src]
+            return {"device_ms": 1.0 + math.log2(d["delta"] / best) ** 2}
+
+    r = A.tune_single(S(None), [1, 2], (50, 100, 200), grid=[dict(l1_pop=32)], repeats=1, confirm=1)
+    assert r["params"]["delta"] == 100 and r["sources"] == [1, 2]
+    assert set(seen_src) == {1, 2}
+    assert math.isclose(r["device_ms"], 2.0)
+
+
+def test_tune_batch_explores_the_l1_ring():
+    # best: a 1024-record ring with the spill threshold at 255
+    def cost(d):
+        c = 5.0
+        c += 0.0 if (d["l1_ring"] or 512) == 1024 else 0.8
+        c += {255: 0.0, 128: 0.3}.get(d["l1_spill"], 0.6)
+        return c
+
+    s = FakeSolver(cost)
+    r = A.tune_batch(s, [0, 1, 2])
+    assert r["params"]["l1_ring"] == 1024 and r["params"]["l1_spill"] == 255
+    # a 512-record ring never ran with a threshold above 64
+    assert all(d["l1_spill"] <= (d["l1_ring"] or 512) - 448 for d in s.seen)
+    assert {d["l1_ring"] or 512 for d in s.seen} == {512, 1024}

@@ -26,7 +26,7 @@ def run(args, timeout=120, **env):
     e = dict(os.environ)
     for k in ("BENCH_SOURCE", "BENCH_DELTA", "BENCH_WARMUPS", "BENCH_REPEATS", "BENCH_QUEUE",
               "MLMQ_WORK_BLOCKS", "L3_SUPPLEMENT_ORACLE", "MLMQ_SKIP_CPU_CHECK",
-              "MLMQ_MULTI_MODE"):
+              "MLMQ_MULTI_MODE", "MLMQ_L2_REPORT"):
         e.pop(k, None)
     e.update({k: str(v) for k, v in env.items()})
     return subprocess.run([CLI, *args], capture_output=True, text=True, env=e, timeout=timeout)
@@ -198,3 +198,39 @@ def test_multi_source_more_sources_than_vertices():
     assert p.returncode == 0, p.stdout + p.stderr
     per = multi_source_report(p.stdout)
     assert per == {k: f"{k % 16} correct" for k in range(20)}
+
+
+def fields(stdout, tag):
+    """The reference audit's field parser (scripts/multigpu/audit_supplement.py:10)."""
+    return [dict(t.split("=", 1) for t in line.split()[1:] if "=" in t)
+            for line in stdout.splitlines() if line.startswith(tag + " ")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("queue", ["L1SLF_L2DQ", "L1V_L2DQ", "L1N_L2DQ"])
+def test_queue_audit_lines(queue):
+    """L2_CAPACITY once, then per query L2_FINAL and (MLMQ_L2_REPORT=1) one
+    CAPACITY_BUCKET line per bucket and a CAPACITY_QUERY line; the checks the
+    reference's capacity audit applies to them hold (audit_supplement.py:28-36,
+    60-67: reads == writes per bucket, total == completed)."""
+    p = run(["-i", FIXTURE], BENCH_SOURCE=1030, BENCH_WARMUPS=1, BENCH_REPEATS=2, BENCH_QUEUE=queue,
+            MLMQ_L2_REPORT=1)
+    assert p.returncode == 0, p.stderr
+    caps = fields(p.stdout, "L2_CAPACITY")
+    assert len(caps) == 1
+    c = caps[0]
+    assert int(c["record_bytes"]) == 8 and int(c["buckets"]) == 16
+    assert int(c["per_bucket"]) * int(c["buckets"]) <= int(c["allocated_records"])
+    finals = fields(p.stdout, "L2_FINAL")
+    assert len(finals) == 3
+    assert all(x["reads"] == x["writes"] == x["completed"] for x in finals)
+    buckets = fields(p.stdout, "CAPACITY_BUCKET")
+    queries = fields(p.stdout, "CAPACITY_QUERY")
+    assert len(buckets) == 3 * 16 and len(queries) == 3
+    for b in buckets:
+        assert b["reads"] == b["writes"] and 0 <= int(b["writes"])
+        assert b["capacity"] == c["per_bucket"]
+    for i, q in enumerate(queries):
+        total = sum(int(b["writes"]) for b in buckets[16 * i:16 * (i + 1)])
+        assert int(q["total"]) == total == int(q["completed"]) == int(finals[i]["writes"])
+    assert all(r["correct"] == "1" for r in bench_rows(p.stdout))

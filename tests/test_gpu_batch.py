@@ -100,6 +100,29 @@ def test_strict_window_batch(solver):
                 buckets=32, bucket_window=1)
 
 
+@pytest.mark.parametrize("waves", [4, 8, 16])
+@pytest.mark.parametrize("ring,spill", [(1024, 255), (1024, 128), (1024, 64), (512, 16)])
+def test_l1_ring_sizes(solver, waves, ring, spill):
+    """Both L1 ring sizes (the LDS segment per wave) at every workgroup size,
+    with spill thresholds up to the largest the ring allows: road-like grid,
+    hub graph and the golden fixture, every row exact."""
+    say(f"l1 ring {ring} spill {spill} waves {waves}")
+    row, col, w = grid_csr(60, 70, seed=41)
+    solver.load_csr(row, col, w)
+    check_batch(solver, row, col, w, [0, 17, 4199, 2100, 999, 0], batch_waves=waves, l1_ring=ring,
+                l1_spill=spill, delta=3000)
+    row, col, w = hub_csr(3000, [2500, 1800], 12000, seed=43)
+    solver.load_csr(row, col, w)
+    check_batch(solver, row, col, w, [0, 1, 5, 2999], batch_waves=waves, l1_ring=ring,
+                l1_spill=spill, delta=400)
+    solver.load_gr(os.path.join(GOLDEN, "fixture.gr"))
+    sources = [0, 1030, 2050, 1030]
+    solver.solve_batch(sources, timeout_s=TIMEOUT, batch_waves=waves, l1_ring=ring, l1_spill=spill)
+    for i, src in enumerate(sources):
+        exp = np.fromfile(os.path.join(GOLDEN, f"s{src}.i32"), dtype=np.int32)
+        assert np.array_equal(solver.batch_dist(i), exp), f"query {i}"
+
+
 def test_graph_replays_interleaved(solver):
     say("graph replays with eager work in between")
     row, col, w = grid_csr(80, 90, 21)

@@ -303,3 +303,25 @@ def test_errors_and_recovery(solver):
     solver.solve_batch(np.array([5, 6], dtype=np.int32), timeout_s=TIMEOUT)
     ref, _, _ = O.dijkstra(row, col, w, 6)
     assert np.array_equal(solver.batch_dist(1), ref)
+
+
+@pytest.mark.parametrize("queue,kw", [("L1SLF_L2DQ", {}), ("L1V_L2DQ", dict(buckets=4, bucket_window=2)),
+                                      ("L1N_L2DQ", dict(bucket_capacity=64))])
+def test_queue_counts_balance(queue, kw):
+    """Per-bucket reservations equal reads after a solve, their sum equals the
+    credits returned and the pushes counted by the kernel; a ring smaller than
+    a bucket's traffic (64 slots, L2-only queue) reuses its slots without
+    losing a record."""
+    print(f"[gpu-test] queue counts {queue} {kw}", flush=True)
+    row, col, w = grid_csr(40, 50, seed=17)
+    ref, _, _ = O.dijkstra(row, col, w, 0)
+    with Solver(0) as s:
+        s.load_csr(row, col, w)
+        t = s.solve(0, queue=queue, delta=2000, timeout_s=20.0, **kw)
+        assert np.array_equal(s.dist(), ref)
+        q = s.queue_counts()
+        assert q["buckets"] == kw.get("buckets", 16)
+        assert q["writes"] == q["reads"]
+        assert sum(q["writes"]) == q["completed"] == t["l2_pushes"]
+        assert q["capacity"] == t["bucket_capacity"]
+        assert q["wrapped"] == sum(x > q["capacity"] for x in q["writes"])

@@ -96,6 +96,74 @@ def test_alone_equals_mix(graphs):
     s.close()
 
 
+@pytest.fixture(scope="module")
+def mixed_graphs():
+    """road264k, a 300 x 400 grid and RMAT-16, each with its own settings."""
+    road = HostGraph.knn(264_346, 3, seed=1, weight_scale=1e5)
+    rmat = HostGraph.rmat(16, 16, seed=9, wmin=1, wmax=1000)
+    gs = [G(*(x.copy() for x in road.arrays())), G(*grid_csr(300, 400, seed=21)),
+          G(*(x.copy() for x in rmat.arrays()))]
+    road.close()
+    rmat.close()
+    # rings large enough that no request overflows (an overflow reruns the
+    # request in a later launch)
+    params = [dict(delta=200_000, bucket_capacity=1 << 16),
+              dict(delta=4000, buckets=4, bucket_window=2, bucket_capacity=1 << 16),
+              dict(delta=50, light_split=1, buckets=64, bucket_window=2, l2_batch=64,
+                   min_read=64, l1_pop=64, bucket_capacity=1 << 16)]
+    return gs, params
+
+
+@pytest.mark.parametrize("spread", [0, 512])
+def test_mixed_graph_launch(mixed_graphs, spread):
+    """One launch holds requests of three graphs of different sizes and
+    settings (light/heavy split on the RMAT graph only): every row equals the
+    oracle's and the row of the same request served alone."""
+    gs, params = mixed_graphs
+    say(f"mixed-graph launch, spread_waves={spread}")
+    s = Server(0, hbm_budget=4 << 30, streams=1, max_batch=64, timeout_s=TIMEOUT,
+               spread_waves=spread)
+    for g, p in zip(gs, params):
+        s.add_graph((g.row, g.col, g.w), **p)
+    rng = np.random.default_rng(31)
+    gsel = np.repeat(np.arange(3, dtype=np.int32), 6)
+    rng.shuffle(gsel)
+    src = np.array([rng.integers(0, gs[g].n) for g in gsel], dtype=np.int32)
+    res, st, mixed = run_checked(s, gs, np.zeros(len(gsel)), gsel, src, "mixed launch")
+    # all 18 requests in one launch
+    diag = (f"launches {st['launches']} launch ids {res['launch'].tolist()} "
+            f"retries {res['retries'].tolist()} batch sizes {res['batch_size'].tolist()}")
+    assert (res["retries"] == 0).all(), diag
+    assert st["launches"] == 1 and (res["launch"] == res["launch"][0]).all(), diag
+    assert (res["batch_size"] == len(gsel)).all(), diag
+    for i in range(len(gsel)):
+        _, st1, alone = run_checked(s, gs, [0.0], [gsel[i]], [src[i]], f"alone {i}")
+        assert st1["launches"] == 1
+        assert np.array_equal(alone[0], mixed[i]), i
+    say(f"mixed launch: device p50 {np.median(res['device_ms']):.2f} ms, "
+        f"latency max {res['latency_ms'].max():.2f} ms")
+    s.close()
+
+
+def test_mixed_widths_split_launches(graphs):
+    """Graphs with different workgroup widths are not mixed: each launch
+    holds requests of one width, and every row is exact."""
+    say("mixed widths")
+    s = Server(0, hbm_budget=256 * MiB, streams=1, max_batch=32, timeout_s=TIMEOUT, spread_waves=0)
+    for g, w in zip(graphs, (4, 8, 4)):
+        s.add_graph((g.row, g.col, g.w), delta=2000 if g.n != 20000 else 200000, waves=w)
+    gsel = np.array([0, 1, 2, 0, 1, 2], dtype=np.int32)
+    src = np.array([1, 2, 3, 4, 5, 6], dtype=np.int32)
+    res, st, _ = run_checked(s, graphs, np.zeros(6), gsel, src, "widths")
+    # graphs 0 and 2 (4 waves) share launches, graph 1 (8 waves) has its own
+    for i in range(6):
+        for j in range(6):
+            if res["launch"][i] == res["launch"][j]:
+                assert (gsel[i] == 1) == (gsel[j] == 1)
+    assert len(set(res["launch"][gsel != 1])) == 1
+    s.close()
+
+
 def test_burst_batches_and_verifies_on_device(graphs):
     say("burst + device verification")
     s, ids = new_server(graphs, streams=2, max_batch=16, verify=1)
