@@ -37,7 +37,22 @@
  *     expected row registered by mlmq_server_set_expected() is compared on the
  *     device and its mismatch count reported.
  *
- * A server is used from one host thread.  Every call is synchronous.
+ * Live serving (mlmq_server_submit / _poll / _wait / _drain / _stop): one
+ * persistent launch serves requests as they are submitted.  The host writes
+ * each admitted request (its graph, slot, settings and number of
+ * workgroups) into a table in pinned host memory and publishes one ticket per
+ * workgroup; workgroups of the running launch take tickets in order, so a
+ * request starts as soon as workgroups free up and completes on its own: its
+ * result is taken while the launch keeps serving others (no request waits
+ * for a launch to end).  Admission is by slot bytes as above.  A service
+ * thread of the server runs the admission, publication and completion loop;
+ * the launch ends after config.live_idle_ms without requests, or at
+ * mlmq_server_stop / _destroy, and starts again on the next submission.
+ * config.mode = MLMQ_SERVE_LIVE makes mlmq_server_run replay its trace
+ * through the same path.
+ *
+ * A server is used from one host thread (live calls may come from several
+ * threads).  Every call except the live ones is synchronous.
  */
 #ifndef MLMQ_SERVE_H
 #define MLMQ_SERVE_H
@@ -51,6 +66,10 @@ extern "C" {
 #endif
 
 #define MLMQ_E_REJECTED (-10)  /* request refused at admission (max_pending waiting) */
+#define MLMQ_E_PENDING  (-11)  /* live request not finished yet (mlmq_server_poll / _wait) */
+
+#define MLMQ_SERVE_LANES 0     /* mlmq_server_run: launches of waiting requests on launch lanes */
+#define MLMQ_SERVE_LIVE  1     /* mlmq_server_run: the persistent launch of live serving */
 
 typedef struct mlmq_server mlmq_server;
 
@@ -66,7 +85,12 @@ typedef struct mlmq_server_config {
                                  that is more than the graph's workgroups x waves: a request
                                  arriving at a quiet server gets much of the device; 0 = off
                                  (default 512) */
-    int32_t  reserved[7];
+    int32_t  mode;            /* mlmq_server_run path: MLMQ_SERVE_LANES (default) or MLMQ_SERVE_LIVE */
+    int32_t  live_waves;      /* live serving: workgroups of the persistent launch (one wave each,
+                                 one of them the dispatcher); 0 = every resident one (up to 4096) */
+    int32_t  live_idle_ms;    /* live serving: the launch ends after this long without requests;
+                                 0 = 2000 */
+    int32_t  reserved[4];
 } mlmq_server_config;
 
 /* Per-graph solver settings of served queries. */
@@ -156,6 +180,23 @@ int mlmq_server_clear_expected(mlmq_server *s);
 /* Replay `nreq` requests in real time; results[i] belongs to reqs[i]. */
 int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq,
                     mlmq_serve_result *results, mlmq_serve_stats *stats);
+
+/* ---- Live serving (no reference counterpart) ----
+ * Submit one request now; `dist_out` (optional, n int32) receives the row
+ * before the request is reported done.  Returns its id in *request_id, or
+ * MLMQ_E_REJECTED when config.max_pending requests already wait. */
+int mlmq_server_submit(mlmq_server *s, int32_t graph, int32_t src, int32_t *dist_out,
+                       int64_t *request_id);
+/* MLMQ_OK with the result once the request is done (latency = when its
+ * result was taken on the host, minus submission), MLMQ_E_PENDING before. */
+int mlmq_server_poll(mlmq_server *s, int64_t request_id, mlmq_serve_result *result);
+/* Waits up to timeout_s (<= 0: no limit) for the request: MLMQ_OK or MLMQ_E_PENDING. */
+int mlmq_server_wait(mlmq_server *s, int64_t request_id, double timeout_s, mlmq_serve_result *result);
+/* Waits until every submitted request is done (MLMQ_OK) or timeout_s passed (MLMQ_E_PENDING). */
+int mlmq_server_drain(mlmq_server *s, double timeout_s);
+/* Ends the persistent launch and the service thread after the submitted
+ * requests are done; a later submission starts them again. */
+int mlmq_server_stop(mlmq_server *s);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,10 @@ ERRORS = {
     -1: "MLMQ_E_INVALID", -2: "MLMQ_E_NOGRAPH", -3: "MLMQ_E_HIP", -4: "MLMQ_E_NOMEM",
     -5: "MLMQ_E_IO", -6: "MLMQ_E_QUEUE_OVERFLOW", -7: "MLMQ_E_TIMEOUT",
     -8: "MLMQ_E_INTERNAL", -9: "MLMQ_E_UNSUPPORTED", -10: "MLMQ_E_REJECTED",
+    -11: "MLMQ_E_PENDING",
 }
+MLMQ_E_PENDING = -11
+SERVE_MODES = {"lanes": 0, "live": 1}
 QUEUE_MODES = {"L1SLF_L2DQ": 19, "L1N_L2DQ": 1, "L1V_L2DQ": 0}
 NOALLOC_RING_SLOTS = 1 << 23   # MLMQ_NOALLOC_RING_SLOTS
 
@@ -42,6 +45,8 @@ SERVE_SYMBOLS = [
     "mlmq_server_destroy", "mlmq_server_last_error", "mlmq_server_add_csr", "mlmq_server_add_gr",
     "mlmq_server_set_graph_params", "mlmq_server_graph_size", "mlmq_server_slot_bytes",
     "mlmq_server_set_expected", "mlmq_server_clear_expected", "mlmq_server_run",
+    "mlmq_server_submit", "mlmq_server_poll", "mlmq_server_wait", "mlmq_server_drain",
+    "mlmq_server_stop",
 ]
 
 
@@ -143,7 +148,9 @@ class ServerConfig(ctypes.Structure):
         ("hbm_budget", ctypes.c_uint64), ("streams", ctypes.c_int32),
         ("max_batch", ctypes.c_int32), ("max_pending", ctypes.c_int32),
         ("verify", ctypes.c_int32), ("timeout_s", ctypes.c_double),
-        ("spread_waves", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("spread_waves", ctypes.c_int32), ("mode", ctypes.c_int32),
+        ("live_waves", ctypes.c_int32), ("live_idle_ms", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -276,6 +283,11 @@ def load() -> ctypes.CDLL:
         "mlmq_server_clear_expected": (ctypes.c_int, [P]),
         "mlmq_server_run": (ctypes.c_int, [P, ctypes.POINTER(ServeRequest), i32,
                                            ctypes.POINTER(ServeResult), ctypes.POINTER(ServeStats)]),
+        "mlmq_server_submit": (ctypes.c_int, [P, i32, i32, P, ctypes.POINTER(i64)]),
+        "mlmq_server_poll": (ctypes.c_int, [P, i64, ctypes.POINTER(ServeResult)]),
+        "mlmq_server_wait": (ctypes.c_int, [P, i64, ctypes.c_double, ctypes.POINTER(ServeResult)]),
+        "mlmq_server_drain": (ctypes.c_int, [P, ctypes.c_double]),
+        "mlmq_server_stop": (ctypes.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

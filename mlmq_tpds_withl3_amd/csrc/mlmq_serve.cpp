@@ -6,9 +6,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -60,6 +62,54 @@ struct Lane {
     std::vector<uint32_t> caps;   // ring capacity (log2) each request ran with
 };
 
+// ---- live serving state ----
+constexpr uint32_t kLiveEntries = 4096;     // request table entries
+constexpr uint32_t kLiveRing = 1u << 16;    // ticket ring (power of two)
+constexpr uint32_t kLiveMaxGrid = 4096;     // persistent workgroups at most
+
+struct LiveReq {
+    int32_t graph = 0, src = 0;
+    int32_t *dist_out = nullptr;
+    double arrival_ms = 0;     // live clock
+    double publish_ms = 0;
+    int32_t retries = 0;
+    int32_t trace_index = -1;  // mlmq_server_run: index into its request array
+    Slot slot;
+    uint32_t cap_log2 = 0;
+    int32_t entry = -1;
+    uint32_t first = 0, parts = 0;
+    bool verified = false;
+    bool done = false;
+    mlmq_serve_result res{};
+};
+
+struct Live {
+    std::mutex mu;                        // guards everything below
+    std::condition_variable cv;           // a request finished
+    std::thread thread;
+    bool thread_on = false, quit = false, thread_done = false;
+    int64_t next_id = 0;
+    std::deque<int64_t> pending;          // admitted in this order
+    std::unordered_map<int64_t, LiveReq> reqs;
+    std::vector<int32_t> inflight;        // entry indices published and not finished
+    std::vector<int64_t> entry_req;       // request id of each entry
+    std::vector<int32_t> free_entries;
+    // device and pinned resources (created with the first launch)
+    bool ready = false, running = false;
+    hipStream_t ks = nullptr, side = nullptr;
+    hipEvent_t ev_end = nullptr;
+    uint32_t *hctl = nullptr;             // pinned host control words
+    uint8_t *entries = nullptr;           // pinned host entry table
+    uint32_t *ring = nullptr;             // pinned host ticket ring
+    uint32_t *dev = nullptr;              // device counters
+    void *sync = nullptr;                 // device per-entry state
+    size_t entry_bytes = 0;
+    uint32_t published = 0, grid = 0, parts_inflight = 0, launches = 0, seq = 0;
+    double idle_since = 0;
+    clk::time_point t0 = clk::now();
+    uint64_t peak_inflight = 0;
+};
+
 }  // namespace
 
 struct mlmq_server {
@@ -72,6 +122,7 @@ struct mlmq_server {
     std::vector<Lane> lanes;
     std::deque<int32_t> pending;   // waiting request ids, in arrival order
     hipEvent_t ev_ref = nullptr;
+    Live live;
     std::string err;
 };
 
@@ -266,6 +317,467 @@ int add_graph_ctx(mlmq_server *s, mlmq_ctx *ctx, const mlmq_serve_graph_params *
     return MLMQ_OK;
 }
 
+double live_now(const Live &L) {
+    return std::chrono::duration<double, std::milli>(clk::now() - L.t0).count();
+}
+
+// Pinned tables, device counters and streams of live serving (once).
+int live_prepare(mlmq_server *s) {
+    Live &L = s->live;
+    if (L.ready) return MLMQ_OK;
+    L.entry_bytes = mlmq::stream_entry_bytes();
+    HIP_TRY(s, hipSetDevice(s->device));
+    HIP_TRY(s, hipStreamCreateWithFlags(&L.ks, hipStreamNonBlocking));
+    HIP_TRY(s, hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+    HIP_TRY(s, hipEventCreateWithFlags(&L.ev_end, hipEventDisableTiming));
+    const unsigned flags = hipHostMallocCoherent | hipHostMallocMapped;
+    HIP_TRY(s, hipHostMalloc((void **)&L.hctl, sizeof(uint32_t) * mlmq::kStreamHostCtlWords, flags));
+    HIP_TRY(s, hipHostMalloc((void **)&L.entries, L.entry_bytes * kLiveEntries, flags));
+    HIP_TRY(s, hipHostMalloc((void **)&L.ring, sizeof(uint32_t) * kLiveRing, flags));
+    std::memset(L.hctl, 0, sizeof(uint32_t) * mlmq::kStreamHostCtlWords);
+    std::memset(L.entries, 0, L.entry_bytes * kLiveEntries);
+    std::memset(L.ring, 0, sizeof(uint32_t) * kLiveRing);
+    HIP_TRY(s, hipMalloc((void **)&L.dev, 64));
+    HIP_TRY(s, hipMalloc(&L.sync, mlmq::stream_sync_bytes() * kLiveEntries));
+    int occ = 0;
+    HIP_TRY(s, mlmq::stream_occupancy(&occ));
+    uint32_t grid = (uint32_t)std::max(occ, 1) * (uint32_t)s->cus;
+    if (s->cfg.live_waves > 0) grid = std::min(grid, (uint32_t)s->cfg.live_waves);
+    L.grid = std::max<uint32_t>(2, std::min(grid, kLiveMaxGrid));
+    L.entry_req.assign(kLiveEntries, -1);
+    L.free_entries.clear();
+    for (int32_t e = (int32_t)kLiveEntries - 1; e >= 0; --e) L.free_entries.push_back(e);
+    L.ready = true;
+    return MLMQ_OK;
+}
+
+void live_release(mlmq_server *s) {
+    Live &L = s->live;
+    if (L.ks) (void)hipStreamSynchronize(L.ks);
+    if (L.side) (void)hipStreamSynchronize(L.side);
+    if (L.hctl) (void)hipHostFree(L.hctl);
+    if (L.entries) (void)hipHostFree(L.entries);
+    if (L.ring) (void)hipHostFree(L.ring);
+    if (L.dev) (void)hipFree(L.dev);
+    if (L.sync) (void)hipFree(L.sync);
+    if (L.ev_end) (void)hipEventDestroy(L.ev_end);
+    if (L.ks) (void)hipStreamDestroy(L.ks);
+    if (L.side) (void)hipStreamDestroy(L.side);
+    L.hctl = nullptr;
+    L.entries = nullptr;
+    L.ring = nullptr;
+    L.dev = nullptr;
+    L.sync = nullptr;
+    L.ev_end = nullptr;
+    L.ks = L.side = nullptr;
+    L.ready = false;
+}
+
+void store_host(uint32_t *p, uint32_t v) {
+    __atomic_store_n(p, v, __ATOMIC_RELEASE);
+}
+
+// Start the persistent launch: device counters and per-entry state zeroed,
+// host control words reset, tickets counted from 0.
+int live_launch(mlmq_server *s) {
+    Live &L = s->live;
+    int rc = live_prepare(s);
+    if (rc) return rc;
+    HIP_TRY(s, hipMemsetAsync(L.dev, 0, 64, L.ks));
+    HIP_TRY(s, hipMemsetAsync(L.sync, 0, mlmq::stream_sync_bytes() * kLiveEntries, L.ks));
+    for (size_t i = 0; i < mlmq::kStreamHostCtlWords; ++i) store_host(L.hctl + i, 0u);
+    L.published = 0;
+    mlmq::StreamLaunch S;
+    S.host_ctl = L.hctl;
+    S.entries = L.entries;
+    S.ring = L.ring;
+    S.ring_size = kLiveRing;
+    S.dev = L.dev;
+    S.sync = L.sync;
+    // a bound on the whole launch: it ends (and is started again on demand)
+    // after ten minutes even if it is never stopped
+    S.lifetime_ticks = (uint64_t)(600.0 * (double)s->wall_khz * 1000.0);
+    S.grid = L.grid;
+    HIP_TRY(s, mlmq::launch_stream(S, L.ks));
+    HIP_TRY(s, hipEventRecord(L.ev_end, L.ks));
+    L.running = true;
+    ++L.launches;
+    L.idle_since = live_now(L);
+    return MLMQ_OK;
+}
+
+// A finished entry: its request's result, or a rerun after an overflow.
+int live_finish(mlmq_server *s, int32_t e, uint32_t status, uint32_t mism, uint64_t t0, uint64_t t1) {
+    Live &L = s->live;
+    const int64_t id = L.entry_req[e];
+    LiveReq &r = L.reqs[id];
+    GraphEntry &g = s->graphs[r.graph];
+    L.entry_req[e] = -1;
+    L.free_entries.push_back(e);
+    L.parts_inflight -= r.parts;
+    r.entry = -1;
+    Slot sl = r.slot;
+    const int st = device_status(status);
+    if (st == MLMQ_E_QUEUE_OVERFLOW && g.cap_log2 < 30) {
+        if (g.cap_log2 == r.cap_log2) {
+            drop_free_slots(s, g);
+            ++g.cap_log2;
+            size_slots(g);
+        }
+        sl.fresh = true;
+        give_back(s, r.graph, sl);
+        ++r.retries;
+        L.pending.push_front(id);
+        return MLMQ_OK;
+    }
+    if (st == MLMQ_OK && r.dist_out) {
+        HIP_TRY(s, hipMemcpyAsync(r.dist_out, s->arena.base + sl.off, sizeof(int32_t) * (size_t)g.v.n,
+                                  hipMemcpyDeviceToHost, L.side));
+        HIP_TRY(s, hipStreamSynchronize(L.side));
+    }
+    r.res.status = st;
+    r.res.retries = r.retries;
+    r.res.latency_ms = live_now(L) - r.arrival_ms;
+    r.res.queue_ms = r.publish_ms - r.arrival_ms;
+    r.res.device_ms = t1 > t0 ? (double)(t1 - t0) / (double)s->wall_khz : 0.0;
+    r.res.mismatches = r.verified ? (int32_t)mism : -1;
+    r.res.launch = (int32_t)L.launches - 1;
+    r.res.batch_size = (int32_t)r.parts;   // workgroups that served it
+    sl.fresh = st != MLMQ_OK;
+    give_back(s, r.graph, sl);
+    r.done = true;
+    L.cv.notify_all();
+    return MLMQ_OK;
+}
+
+// Fail a request that cannot run (no slot will ever fit).
+void live_fail(mlmq_server *s, int64_t id, int code) {
+    Live &L = s->live;
+    LiveReq &r = L.reqs[id];
+    r.res.status = code;
+    r.res.mismatches = -1;
+    r.res.latency_ms = live_now(L) - r.arrival_ms;
+    r.done = true;
+    L.cv.notify_all();
+}
+
+// Stop the persistent launch: serve what is published, then wait for it.
+int live_halt(mlmq_server *s) {
+    Live &L = s->live;
+    if (!L.running) return MLMQ_OK;
+    store_host(L.hctl + 1, 1u);
+    HIP_TRY(s, hipEventSynchronize(L.ev_end));
+    L.running = false;
+    return MLMQ_OK;
+}
+
+// One pass of the live loop (caller holds L.mu).  Returns <0 on a HIP error,
+// 1 if anything happened, 0 if not.
+int live_step(mlmq_server *s) {
+    Live &L = s->live;
+    int acted = 0;
+    // completions
+    for (size_t i = 0; i < L.inflight.size();) {
+        const int32_t e = L.inflight[i];
+        uint32_t done = 0, status = 0, mism = 0;
+        uint64_t t0 = 0, t1 = 0;
+        mlmq::stream_entry_read(L.entries + L.entry_bytes * (size_t)e, &done, &status, &mism, &t0, &t1);
+        if (!done) {
+            ++i;
+            continue;
+        }
+        L.inflight[i] = L.inflight.back();
+        L.inflight.pop_back();
+        const int rc = live_finish(s, e, status, mism, t0, t1);
+        if (rc) return rc;
+        acted = 1;
+    }
+    // the launch ended by itself (its lifetime): what it did not start is
+    // published again to the next launch
+    if (L.running && hipEventQuery(L.ev_end) == hipSuccess) {
+        L.running = false;
+        const uint32_t served = __atomic_load_n(L.hctl + 2, __ATOMIC_ACQUIRE);
+        for (size_t i = 0; i < L.inflight.size();) {
+            const int32_t e = L.inflight[i];
+            const int64_t id = L.entry_req[e];
+            LiveReq &r = L.reqs[id];
+            uint32_t done = 0, status = 0, mism = 0;
+            uint64_t t0 = 0, t1 = 0;
+            mlmq::stream_entry_read(L.entries + L.entry_bytes * (size_t)e, &done, &status, &mism, &t0, &t1);
+            if (done) {
+                ++i;
+                continue;   // finished after all: collected on the next pass
+            }
+            L.inflight[i] = L.inflight.back();
+            L.inflight.pop_back();
+            L.entry_req[e] = -1;
+            L.free_entries.push_back(e);
+            L.parts_inflight -= r.parts;
+            Slot sl = r.slot;
+            if ((int32_t)(r.first - served) >= 0) {
+                give_back(s, r.graph, sl);     // never touched by the device
+                L.pending.push_front(id);
+            } else {
+                sl.fresh = true;               // started but unfinished
+                give_back(s, r.graph, sl);
+                live_fail(s, id, MLMQ_E_TIMEOUT);
+            }
+        }
+        acted = 1;
+    }
+    if (!L.pending.empty() && !L.running) {
+        const int rc = live_launch(s);
+        if (rc) return rc;
+        acted = 1;
+    }
+    // admission and publication, in queue order
+    const uint64_t timeout_ticks = (uint64_t)(s->cfg.timeout_s * (double)s->wall_khz * 1000.0);
+    while (L.running && !L.pending.empty() && !L.free_entries.empty()) {
+        const int64_t id = L.pending.front();
+        LiveReq &r = L.reqs[id];
+        GraphEntry &g = s->graphs[r.graph];
+        const uint32_t cap = L.grid - 1;
+        uint32_t parts = std::min<uint32_t>((uint32_t)g.p.workgroups * (uint32_t)g.p.waves, cap);
+        if (s->cfg.spread_waves > 0) {
+            // few requests running: spread this one over the idle workgroups
+            const uint32_t idle = cap > L.parts_inflight ? cap - L.parts_inflight : 0;
+            const uint32_t share = std::min<uint32_t>(
+                std::min<uint32_t>((uint32_t)s->cfg.spread_waves, kMaxSpread),
+                idle / (uint32_t)std::max<size_t>(1, L.pending.size()));
+            parts = std::max(parts, std::min(share, cap));
+        }
+        uint32_t oldest = L.published;
+        for (int32_t e : L.inflight) oldest = std::min(oldest, L.reqs[L.entry_req[e]].first);
+        if (L.published + parts - oldest > kLiveRing) break;
+        Slot sl;
+        if (!take_slot(s, r.graph, &sl)) {
+            if (L.inflight.empty()) {   // nothing runs and it does not fit: it never will
+                L.pending.pop_front();
+                live_fail(s, id, MLMQ_E_NOMEM);
+                acted = 1;
+                continue;
+            }
+            break;
+        }
+        const int32_t e = L.free_entries.back();
+        L.free_entries.pop_back();
+        uint8_t *b = s->arena.base + sl.off;
+        mlmq::SolverLaunch Ls = graph_launch(g, timeout_ticks);
+        Ls.dist = (uint32_t *)b;
+        Ls.ctl = (uint32_t *)(b + g.row_bytes);
+        Ls.slots = (uint64_t *)(b + g.row_bytes + g.ctl_bytes);
+        const uint32_t *expect = nullptr;
+        if (s->cfg.verify) {
+            auto it = g.expected.find(r.src);
+            if (it != g.expected.end()) expect = it->second;
+        }
+        mlmq::stream_entry_write(L.entries + L.entry_bytes * (size_t)e, Ls, (uint32_t)r.src, sl.fresh, expect,
+                                 L.published, parts, ++L.seq);
+        for (uint32_t j = 0; j < parts; ++j) L.ring[(L.published + j) & (kLiveRing - 1)] = (uint32_t)e;
+        r.slot = sl;
+        r.cap_log2 = g.cap_log2;
+        r.entry = e;
+        r.first = L.published;
+        r.parts = parts;
+        r.verified = expect != nullptr;
+        r.publish_ms = live_now(L);
+        L.published += parts;
+        store_host(L.hctl, L.published);   // release: entry and ring words first
+        L.entry_req[e] = id;
+        L.inflight.push_back(e);
+        L.parts_inflight += parts;
+        L.peak_inflight = std::max<uint64_t>(L.peak_inflight, L.inflight.size());
+        L.pending.pop_front();
+        acted = 1;
+    }
+    // idle: end the launch
+    const double now = live_now(L);
+    if (acted || !L.pending.empty() || !L.inflight.empty()) {
+        L.idle_since = now;
+    } else if (L.running) {
+        const double idle_ms = s->cfg.live_idle_ms > 0 ? s->cfg.live_idle_ms : 2000.0;
+        if (now - L.idle_since > idle_ms) {
+            const int rc = live_halt(s);
+            if (rc) return rc;
+        }
+    }
+    return acted;
+}
+
+void live_thread(mlmq_server *s) {
+    Live &L = s->live;
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(L.mu);
+    for (;;) {
+        const int rc = live_step(s);
+        if (rc < 0) {
+            // a HIP error: every request that waits or runs fails with it
+            for (auto &kv : L.reqs)
+                if (!kv.second.done) live_fail(s, kv.first, rc);
+            L.pending.clear();
+            L.inflight.clear();
+            break;
+        }
+        if (L.quit && L.pending.empty() && L.inflight.empty()) break;
+        lk.unlock();
+        if (rc == 0) std::this_thread::sleep_for(std::chrono::microseconds(L.inflight.empty() ? 200 : 20));
+        lk.lock();
+    }
+    (void)live_halt(s);
+    L.thread_done = true;
+}
+
+int live_stop_thread(mlmq_server *s) {
+    Live &L = s->live;
+    {
+        std::lock_guard<std::mutex> lk(L.mu);
+        if (!L.thread_on) return MLMQ_OK;
+        L.quit = true;
+    }
+    L.thread.join();
+    std::lock_guard<std::mutex> lk(L.mu);
+    L.thread_on = false;
+    L.thread_done = false;
+    L.quit = false;
+    return MLMQ_OK;
+}
+
+// A new request (caller holds L.mu); arrival on the live clock.
+int64_t live_add(mlmq_server *s, int32_t graph, int32_t src, int32_t *dist_out, double arrival_ms) {
+    Live &L = s->live;
+    const int64_t id = L.next_id++;
+    LiveReq &r = L.reqs[id];
+    r.graph = graph;
+    r.src = src;
+    r.dist_out = dist_out;
+    r.arrival_ms = arrival_ms;
+    L.pending.push_back(id);
+    return id;
+}
+
+void fill_stats(const mlmq_serve_result *results, int32_t nreq, uint64_t rejected, uint64_t launches,
+                double wall_ms, double last_arrival_s, double batch_sum, uint64_t peak_bytes,
+                uint64_t peak_inflight, mlmq_serve_stats *stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    std::vector<double> lat, que;
+    double dev = 0;
+    for (int32_t i = 0; i < nreq; ++i) {
+        const mlmq_serve_result &r = results[i];
+        if (r.status == MLMQ_E_REJECTED) continue;
+        if (r.status != MLMQ_OK) {
+            ++stats->failed;
+            continue;
+        }
+        ++stats->completed;
+        lat.push_back(r.latency_ms);
+        que.push_back(r.queue_ms);
+        dev += r.device_ms;
+        if (r.mismatches >= 0) {
+            ++stats->verified;
+            if (r.mismatches > 0) ++stats->mismatched;
+        }
+        stats->retries += (uint64_t)r.retries;
+    }
+    stats->requests = (uint64_t)nreq;
+    stats->rejected = rejected;
+    stats->launches = launches;
+    stats->wall_s = wall_ms / 1000.0;
+    stats->offered_qps = nreq > 0 && last_arrival_s > 0 ? (double)nreq / last_arrival_s : 0.0;
+    stats->completed_qps = wall_ms > 0 ? (double)stats->completed / stats->wall_s : 0.0;
+    stats->p50_ms = pct(lat, 0.50);
+    stats->p99_ms = pct(lat, 0.99);
+    stats->max_ms = lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end());
+    double sum = 0;
+    for (double x : lat) sum += x;
+    stats->mean_ms = lat.empty() ? 0.0 : sum / (double)lat.size();
+    stats->queue_p50_ms = pct(que, 0.50);
+    stats->queue_p99_ms = pct(que, 0.99);
+    stats->mean_batch = launches ? batch_sum / (double)launches : 0.0;
+    stats->mean_device_ms = stats->completed ? dev / (double)stats->completed : 0.0;
+    stats->peak_bytes = peak_bytes;
+    stats->peak_inflight = peak_inflight;
+}
+
+// mlmq_server_run through the live path, on the calling thread.
+int run_live(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq, mlmq_serve_result *results,
+             mlmq_serve_stats *stats) {
+    int rc = live_stop_thread(s);
+    if (rc) return rc;
+    Live &L = s->live;
+    std::unique_lock<std::mutex> lk(L.mu);
+    L.reqs.clear();
+    L.pending.clear();
+    const uint64_t peak_before = s->arena.used;
+    s->arena.peak = s->arena.used;
+    L.peak_inflight = 0;
+    const uint32_t launches0 = L.launches;
+    const double t0 = live_now(L);
+    std::vector<int64_t> ids((size_t)nreq, -1);
+    int32_t next = 0, pending = 0;
+    uint64_t rejected = 0;
+    size_t finished = 0;
+    double wall_ms = 0;
+    while (rc == MLMQ_OK) {
+        const double t = live_now(L) - t0;
+        while (next < nreq && 1000.0 * reqs[next].arrival_s <= t) {
+            pending = (int32_t)L.pending.size();
+            if (s->cfg.max_pending > 0 && pending >= s->cfg.max_pending) {
+                ++rejected;
+            } else {
+                ids[next] = live_add(s, reqs[next].graph, reqs[next].src, reqs[next].dist_out,
+                                     t0 + 1000.0 * reqs[next].arrival_s);
+                L.reqs[ids[next]].trace_index = next;
+            }
+            ++next;
+        }
+        const int st = live_step(s);
+        if (st < 0) {
+            rc = st;
+            break;
+        }
+        finished = 0;
+        for (int32_t i = 0; i < next; ++i)
+            if (ids[i] >= 0 && L.reqs[ids[i]].done) ++finished;
+        if (next == nreq && finished + rejected == (size_t)nreq) break;
+        if (st == 0) {
+            lk.unlock();
+            if (L.inflight.empty() && L.pending.empty() && next < nreq) {
+                const double wait = 1000.0 * reqs[next].arrival_s - (live_now(L) - t0);
+                if (wait > 0.5)
+                    std::this_thread::sleep_for(std::chrono::microseconds((int64_t)((wait - 0.3) * 1000.0)));
+            } else {
+                std::this_thread::yield();
+            }
+            lk.lock();
+        }
+    }
+    {
+        // no idle launch is left behind (the next run starts one again)
+        const int hrc = live_halt(s);
+        if (!rc) rc = hrc;
+    }
+    if (rc) return rc;
+    for (int32_t i = 0; i < nreq; ++i) {
+        mlmq_serve_result &r = results[i];
+        if (ids[i] < 0) {
+            std::memset(&r, 0, sizeof(r));
+            r.status = MLMQ_E_REJECTED;
+            r.mismatches = -1;
+            r.launch = -1;
+            continue;
+        }
+        r = L.reqs[ids[i]].res;
+        wall_ms = std::max(wall_ms, L.reqs[ids[i]].arrival_ms - t0 + r.latency_ms);
+    }
+    L.reqs.clear();
+    if (stats) {
+        const double last = nreq ? reqs[nreq - 1].arrival_s : 0.0;
+        fill_stats(results, nreq, rejected, L.launches - launches0, wall_ms, last, 0.0,
+                   std::max(peak_before, s->arena.peak), L.peak_inflight, stats);
+    }
+    return MLMQ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -280,6 +792,9 @@ void mlmq_server_config_default(mlmq_server_config *cfg) {
     cfg->verify = 1;
     cfg->timeout_s = 60.0;
     cfg->spread_waves = 512;
+    cfg->mode = MLMQ_SERVE_LANES;
+    cfg->live_waves = 0;
+    cfg->live_idle_ms = 2000;
 }
 
 void mlmq_serve_graph_params_default(mlmq_serve_graph_params *p) {
@@ -307,8 +822,10 @@ int mlmq_server_create(int device, const mlmq_server_config *cfg, mlmq_server **
     if (c.streams == 0) c.streams = 4;
     if (c.max_batch == 0) c.max_batch = 64;
     if (c.timeout_s <= 0) c.timeout_s = 60.0;
+    if (c.live_idle_ms == 0) c.live_idle_ms = 2000;
     if (c.streams < 1 || c.streams > 32 || c.max_batch < 1 || c.max_batch > 4096 || c.max_pending < 0 ||
-        c.spread_waves < 0 || c.spread_waves > 65536)
+        c.spread_waves < 0 || c.spread_waves > 65536 ||
+        (c.mode != MLMQ_SERVE_LANES && c.mode != MLMQ_SERVE_LIVE) || c.live_waves < 0 || c.live_idle_ms < 0)
         return MLMQ_E_INVALID;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return MLMQ_E_INVALID;
@@ -347,7 +864,13 @@ int mlmq_server_create(int device, const mlmq_server_config *cfg, mlmq_server **
 
 int mlmq_server_destroy(mlmq_server *s) {
     if (!s) return MLMQ_OK;
+    (void)live_stop_thread(s);
     (void)hipSetDevice(s->device);
+    {
+        std::lock_guard<std::mutex> lk(s->live.mu);
+        (void)live_halt(s);
+    }
+    live_release(s);
     for (Lane &l : s->lanes) {
         if (l.s) (void)hipStreamSynchronize(l.s);
         if (l.ev_solve) (void)hipEventDestroy(l.ev_solve);
@@ -471,6 +994,15 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         if (r.src < 0 || r.src >= s->graphs[r.graph].v.n) return fail(s, MLMQ_E_INVALID, "source out of range");
     }
     HIP_TRY(s, hipSetDevice(s->device));
+    if (s->cfg.mode == MLMQ_SERVE_LIVE) return run_live(s, reqs, nreq, results, stats);
+    {
+        // the launch lanes and live serving share the slot arena: no live launch may run meanwhile
+        int rc0 = live_stop_thread(s);
+        if (rc0) return rc0;
+        std::lock_guard<std::mutex> lk(s->live.mu);
+        rc0 = live_halt(s);
+        if (rc0) return rc0;
+    }
     for (int32_t i = 0; i < nreq; ++i) {
         mlmq_serve_result &r = results[i];
         std::memset(&r, 0, sizeof(r));
@@ -719,47 +1251,86 @@ int mlmq_server_run(mlmq_server *s, const mlmq_serve_request *reqs, int32_t nreq
         return rc;
     }
 
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        std::vector<double> lat, que;
-        double dev = 0;
-        for (int32_t i = 0; i < nreq; ++i) {
-            const mlmq_serve_result &r = results[i];
-            if (r.status == MLMQ_E_REJECTED) continue;
-            if (r.status != MLMQ_OK) {
-                ++stats->failed;
-                continue;
-            }
-            ++stats->completed;
-            lat.push_back(r.latency_ms);
-            que.push_back(r.queue_ms);
-            dev += r.device_ms;
-            if (r.mismatches >= 0) {
-                ++stats->verified;
-                if (r.mismatches > 0) ++stats->mismatched;
-            }
-            stats->retries += (uint64_t)r.retries;
-        }
-        stats->requests = (uint64_t)nreq;
-        stats->rejected = rejected;
-        stats->launches = (uint64_t)launches;
-        stats->wall_s = wall_ms / 1000.0;
-        stats->offered_qps = nreq > 0 && last > 0 ? (double)nreq / last : 0.0;
-        stats->completed_qps = wall_ms > 0 ? (double)stats->completed / stats->wall_s : 0.0;
-        stats->p50_ms = pct(lat, 0.50);
-        stats->p99_ms = pct(lat, 0.99);
-        stats->max_ms = lat.empty() ? 0.0 : *std::max_element(lat.begin(), lat.end());
-        double sum = 0;
-        for (double x : lat) sum += x;
-        stats->mean_ms = lat.empty() ? 0.0 : sum / (double)lat.size();
-        stats->queue_p50_ms = pct(que, 0.50);
-        stats->queue_p99_ms = pct(que, 0.99);
-        stats->mean_batch = launches ? (double)batch_sum / (double)launches : 0.0;
-        stats->mean_device_ms = stats->completed ? dev / (double)stats->completed : 0.0;
-        stats->peak_bytes = std::max(peak_before, s->arena.peak);
-        stats->peak_inflight = peak_inflight;
+    if (stats)
+        fill_stats(results, nreq, rejected, (uint64_t)launches, wall_ms, last, (double)batch_sum,
+                   std::max(peak_before, s->arena.peak), peak_inflight, stats);
+    return MLMQ_OK;
+}
+
+int mlmq_server_submit(mlmq_server *s, int32_t graph, int32_t src, int32_t *dist_out, int64_t *request_id) {
+    if (!s || !request_id) return MLMQ_E_INVALID;
+    if (graph < 0 || graph >= (int32_t)s->graphs.size()) return fail(s, MLMQ_E_INVALID, "unknown graph");
+    if (src < 0 || src >= s->graphs[graph].v.n) return fail(s, MLMQ_E_INVALID, "source out of range");
+    Live &L = s->live;
+    std::lock_guard<std::mutex> lk(L.mu);
+    if (s->cfg.max_pending > 0 && (int32_t)L.pending.size() >= s->cfg.max_pending)
+        return fail(s, MLMQ_E_REJECTED, "too many requests waiting");
+    if (L.thread_on && L.thread_done) {   // the service thread stopped on an error
+        L.thread.join();
+        L.thread_on = false;
+        L.thread_done = false;
+    }
+    *request_id = live_add(s, graph, src, dist_out, live_now(L));
+    if (!L.thread_on) {
+        L.quit = false;
+        L.thread = std::thread(live_thread, s);
+        L.thread_on = true;
     }
     return MLMQ_OK;
+}
+
+int mlmq_server_poll(mlmq_server *s, int64_t request_id, mlmq_serve_result *result) {
+    if (!s) return MLMQ_E_INVALID;
+    Live &L = s->live;
+    std::lock_guard<std::mutex> lk(L.mu);
+    auto it = L.reqs.find(request_id);
+    if (it == L.reqs.end()) return fail(s, MLMQ_E_INVALID, "unknown request id");
+    if (!it->second.done) return MLMQ_E_PENDING;
+    if (result) *result = it->second.res;
+    L.reqs.erase(it);
+    return MLMQ_OK;
+}
+
+int mlmq_server_wait(mlmq_server *s, int64_t request_id, double timeout_s, mlmq_serve_result *result) {
+    if (!s) return MLMQ_E_INVALID;
+    Live &L = s->live;
+    std::unique_lock<std::mutex> lk(L.mu);
+    auto done = [&]() {
+        auto it = L.reqs.find(request_id);
+        return it == L.reqs.end() || it->second.done;
+    };
+    if (timeout_s > 0) {
+        if (!L.cv.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return MLMQ_E_PENDING;
+    } else {
+        L.cv.wait(lk, done);
+    }
+    auto it = L.reqs.find(request_id);
+    if (it == L.reqs.end()) return fail(s, MLMQ_E_INVALID, "unknown request id");
+    if (result) *result = it->second.res;
+    L.reqs.erase(it);
+    return MLMQ_OK;
+}
+
+int mlmq_server_drain(mlmq_server *s, double timeout_s) {
+    if (!s) return MLMQ_E_INVALID;
+    Live &L = s->live;
+    std::unique_lock<std::mutex> lk(L.mu);
+    auto all_done = [&]() {
+        for (auto &kv : L.reqs)
+            if (!kv.second.done) return false;
+        return true;
+    };
+    if (timeout_s > 0) {
+        if (!L.cv.wait_for(lk, std::chrono::duration<double>(timeout_s), all_done)) return MLMQ_E_PENDING;
+    } else {
+        L.cv.wait(lk, all_done);
+    }
+    return MLMQ_OK;
+}
+
+int mlmq_server_stop(mlmq_server *s) {
+    if (!s) return MLMQ_E_INVALID;
+    return live_stop_thread(s);
 }
 
 }  // extern "C"

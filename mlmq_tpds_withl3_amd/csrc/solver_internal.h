@@ -110,6 +110,30 @@ struct ServeLaunch {
     bool ext;                 // some request needs light_split / defer_heavy / strict_window
 };
 
+// Live serving through a request ring (sssp_stream_kernel): entries and the
+// ring live in pinned host memory; the layouts are private to
+// sssp_kernels.hip.
+size_t stream_entry_bytes();
+size_t stream_sync_bytes();
+// Host control block of the ring: {published, stop, stopped_at, alive}.
+constexpr size_t kStreamHostCtlWords = 4;
+void stream_entry_write(void *entry, const SolverLaunch &L, uint32_t src, bool fresh, const uint32_t *expect,
+                        uint32_t first, uint32_t parts, uint32_t seq);
+// done = 0 while the request runs; the other outputs are set once done = 1.
+void stream_entry_read(const void *entry, uint32_t *done, uint32_t *status, uint32_t *mismatches,
+                       uint64_t *t_start, uint64_t *t_end);
+struct StreamLaunch {
+    void *host_ctl;           // pinned host: kStreamHostCtlWords words
+    void *entries;            // pinned host: entry table
+    const uint32_t *ring;     // pinned host: ring_size words
+    uint32_t ring_size;       // power of two
+    uint32_t *dev;            // device: 3 words, zeroed before the launch
+    void *sync;               // device: one StreamSync per entry, zeroed before the launch
+    uint64_t lifetime_ticks;
+    uint32_t grid;            // one dispatcher + serving workgroups (one wave each)
+};
+hipError_t stream_occupancy(int *blocks_per_cu);
+hipError_t launch_stream(const StreamLaunch &S, hipStream_t s);
 hipError_t launch_serve(const ServeLaunch &S, hipStream_t s);
 hipError_t launch_serve_verify(const ServeLaunch &S, hipStream_t s);
 hipError_t batch_occupancy(int waves, int l1_cap, int *blocks_per_cu);

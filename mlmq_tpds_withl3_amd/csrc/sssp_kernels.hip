@@ -17,6 +17,8 @@
 // solve completes even if only part of the grid is resident.
 //
 // S = evidence/l3_supplement_20260921_v2/single_source/ (paths above are in S).
+#include <atomic>
+
 #include "hop_block.h"
 #include "l1_slf.h"
 #include "solver_internal.h"
@@ -869,6 +871,268 @@ __global__ __launch_bounds__(256) void serve_verify_kernel(ServeRequest *reqs) {
     if (lane_id() == 0 && bad) atomicAdd(&r->mismatches, bad);
 }
 
+// ---- live serving: one persistent launch fed through a request ring ----
+//
+// The host keeps a table of request entries and a ring of tickets in pinned,
+// coherent host memory.  To submit a request it writes the entry (graph,
+// slot, settings, source, number of parts k), names the entry in the ring
+// for tickets [first, first + k), then advances `published` by k.  Block 0
+// of the launch is a dispatcher: it mirrors `published` and the host's stop
+// flag into device memory, so that the waiting workgroups poll device memory
+// instead of the bus.  Every other workgroup takes the next ticket from a
+// device counter, waits until it is published, and serves part
+// (ticket - first) of that ticket's request:
+//   1. fill its share of the request's distance row (and of the queue
+//      partition if fresh); part 0 also clears the control block and places
+//      the seed;
+//   2. wait until all k parts have filled their shares (tickets are handed
+//      out in order, so every part of a published request is claimed as soon
+//      as workgroups free up; no request waits on a later one);
+//   3. run the worker loop on the request's partition (part 0's first wave
+//      runs the quiescence check);
+//   4. compare its share of the row with the expected row, if registered;
+//   5. count itself finished; the last part writes status, mismatch count
+//      and clock ticks into the host entry, then its `done` word.
+// A request thus completes, and its result can be taken, while the launch
+// keeps serving others, and a request submitted while the launch runs joins
+// it without a new launch.
+//
+// Slot reuse inside one launch: the parts write the row with plain stores, so
+// before a part counts its share done it releases at agent scope (which
+// writes back its XCD's L2), and after the barrier every part acquires at
+// agent scope (which drops its XCD's stale lines).  The pre-check of the
+// relaxation loop reads distances through the cache and relies on them only
+// ever falling; a line left from the previous request on the same slot would
+// break that, and the acquire removes it.
+//
+// Every wait is bounded: ticket waits by the stop flag and the launch's
+// lifetime, the barrier and the worker loop by the request's watchdog, the
+// dispatcher by the lifetime.  A request whose parts do not all arrive before
+// its watchdog fires completes with the timeout error.
+struct StreamEntry {
+    ServeRequest r;           // graph, slot and settings; status / mismatches / ticks written back
+    uint32_t first;           // first ticket of the request
+    uint32_t parts;           // workgroups serving it
+    uint32_t done;            // 0 while pending; 1 when the result is written
+    uint32_t seq;             // host's sequence number of the request (diagnostics)
+};
+constexpr uint32_t kStreamEntryWords = sizeof(StreamEntry) / 4;
+static_assert(sizeof(StreamEntry) % 8 == 0, "entry words");
+
+struct StreamHostCtl {        // host memory, written by the host
+    uint32_t published;       // tickets published
+    uint32_t stop;            // 1: serve what is published, then exit
+    uint32_t stopped_at;      // written by the dispatcher: tickets below it were served
+    uint32_t alive;           // written by the dispatcher: 1 while running
+};
+
+struct StreamSync {           // device memory, one per entry; zero between requests
+    uint32_t arrive;          // parts whose share of the reset is done
+    uint32_t finished;        // parts done
+    uint32_t mismatches;
+    uint32_t status;
+    uint64_t t_start;
+};
+
+struct StreamArgs {
+    StreamHostCtl *host;      // host memory
+    StreamEntry *entries;     // host memory
+    const uint32_t *ring;     // host memory: entry index of ticket t at t % ring_size
+    uint32_t ring_size;       // power of two
+    uint32_t *dev;            // device: [0] claimed, [1] published mirror, [2] stop mirror
+    StreamSync *sync;         // device: per entry
+    uint64_t lifetime_ticks;  // wall-clock bound of the whole launch
+};
+
+__device__ __forceinline__ uint32_t ld_sys(const uint32_t *p) {
+    return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys_release(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+constexpr uint32_t kNoTicket = 0xFFFFFFFFu;
+
+__device__ void stream_dispatch(const StreamArgs &s, uint64_t end) {
+    if (threadIdx.x != 0) return;
+    st_sys(&s.host->alive, 1u);
+    uint32_t last = 0;
+    for (;;) {
+        // relaxed: the workgroups read the entries with loads that bypass the
+        // caches, after this value; no cache needs invalidating here
+        const uint32_t pub = ld_sys(&s.host->published);
+        if (pub != last) {
+            __hip_atomic_store(s.dev + 1, pub, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            last = pub;
+        }
+        if (ld_sys(&s.host->stop) || wall_clock64() > end) {
+            // tickets below `last` are (or will be) served; later ones are not
+            const uint32_t pub2 = ld_sys(&s.host->published);
+            if (pub2 != last) {
+                __hip_atomic_store(s.dev + 1, pub2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                last = pub2;
+            }
+            st_sys(&s.host->stopped_at, last);
+            __hip_atomic_store(s.dev + 2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(32);
+    }
+    st_sys_release(&s.host->alive, 0u);
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(kWaves * 64) void sssp_stream_kernel(StreamArgs s) {
+    __shared__ uint64_t s_ring[kWaves][kL1Alloc];
+    __shared__ uint32_t s_map[kWaves][kWave];
+    __shared__ uint64_t s_stage[kWaves][kWave];
+    __shared__ uint64_t s_entry64[kStreamEntryWords / 2];   // 8-byte aligned copy of the entry
+    __shared__ uint32_t s_ticket, s_index, s_go;
+    uint32_t *s_entry = reinterpret_cast<uint32_t *>(s_entry64);
+
+    const uint64_t end = wall_clock64() + s.lifetime_ticks;
+    if (blockIdx.x == 0) {
+        stream_dispatch(s, end);
+        return;
+    }
+    const uint32_t w = threadIdx.x / kWave;
+    WaveStats st;
+    st.clear();
+    PhaseClock<false> clk;
+    for (;;) {
+        // 1. a ticket, published
+        if (threadIdx.x == 0) {
+            uint32_t t = fetch_add(s.dev, 1u);
+            uint32_t nap = 1;
+            for (;;) {
+                const uint32_t pub = ld_relaxed(s.dev + 1);
+                if ((int32_t)(t - pub) < 0) break;
+                if (ld_relaxed(s.dev + 2) || wall_clock64() > end) {
+                    t = kNoTicket;
+                    break;
+                }
+                for (uint32_t k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(8);
+                nap = nap < 128 ? nap * 2 : 128;   // at most ~30 us between polls
+            }
+            s_ticket = t;
+            if (t != kNoTicket) s_index = ld_sys(s.ring + (t & (s.ring_size - 1)));
+        }
+        __syncthreads();
+        const uint32_t t = s_ticket;
+        if (t == kNoTicket) break;
+        const uint32_t ei = s_index;
+        // 2. the request entry, copied from host memory (loads that bypass the caches)
+        StreamEntry *E = s.entries + ei;
+        for (uint32_t i = threadIdx.x; i < kStreamEntryWords; i += blockDim.x)
+            s_entry[i] = ld_sys(reinterpret_cast<const uint32_t *>(E) + i);
+        __syncthreads();
+        const StreamEntry &se = *reinterpret_cast<const StreamEntry *>(s_entry);
+        const SolveArgs a = se.r.a;
+        const uint32_t k = se.parts, part = t - se.first;
+        StreamSync *sy = s.sync + ei;
+        const uint64_t deadline = wall_clock64() + a.timeout_ticks;
+        if (part == 0 && threadIdx.x == 0)
+            __hip_atomic_store(&sy->t_start, (uint64_t)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // 3. this part's share of the reset
+        {
+            const uint64_t n = a.n, lo = n * part / k, hi = n * (part + 1) / k;
+            for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+                a.dist[i] = i == se.r.src ? 0u : kUnreached;
+            if (se.r.fresh) {
+                const uint64_t ps = (uint64_t)a.q.nbuckets << a.q.cap_log2;
+                const uint64_t slo = ps * part / k, shi = ps * (part + 1) / k;
+                for (uint64_t i = (slo ? slo : 1) + threadIdx.x; i < shi; i += blockDim.x) a.q.slots[i] = kEmpty;
+            }
+            if (part == 0) {
+                for (uint32_t i = threadIdx.x; i < kStatsOff; i += blockDim.x)
+                    a.q.ctl[i] = i == kTailOff ? 1u : 0u;
+                if (threadIdx.x == 0) a.q.slots[0] = make_rec(se.r.src, 0u);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __hip_atomic_fetch_add(&sy->arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t go = 1, nap = 1;
+            while (ld_relaxed(&sy->arrive) < k) {
+                if (wall_clock64() > deadline) {
+                    go = 0;
+                    break;
+                }
+                for (uint32_t z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(4);
+                nap = nap < 32 ? nap * 2 : 32;
+            }
+            s_go = go;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // 4. the worker loop
+        if (s_go) {
+            run_worker<true, false, false, true, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w],
+                                                     st, clk, deadline);
+        } else if (threadIdx.x == 0) {
+            cas32(&sy->status, kErrNone, kErrTimeout);
+            raise_error(a.q, kErrTimeout);
+        }
+        __threadfence();
+        __syncthreads();
+        // 5. verification of this part's share
+        const uint32_t err = uniform(ld_relaxed(a.q.ctl + kErrOff));
+        if (s_go && err == kErrNone && se.r.expect) {
+            const uint64_t n = a.n, lo = n * part / k, hi = n * (part + 1) / k;
+            uint32_t bad = 0;
+            for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) bad += a.dist[i] != se.r.expect[i] ? 1u : 0u;
+            bad = wave_sum(bad);
+            if (lane_id() == 0 && bad) atomicAdd(&sy->mismatches, bad);
+        }
+        // 6. finish
+        if (threadIdx.x == 0 && part == 0) {
+            uint32_t e = err;
+            if (e == kErrNone) {
+                uint32_t tt = 0, kk = 0;
+                for (uint32_t i = 0; i < a.q.nbuckets; ++i) {
+                    tt += ld_relaxed(a.q.ctl + kTailOff + i * kStride);
+                    kk += ld_relaxed(a.q.ctl + kTakenOff + i * kStride);
+                }
+                if (tt != kk || tt != ld_relaxed(a.q.ctl + kDoneOff)) e = kErrInternal;
+            }
+            cas32(&sy->status, kErrNone, e);
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t before =
+                __hip_atomic_fetch_add(&sy->finished, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (before == k - 1) {
+                // the last part: result into the host entry, then `done`
+                const uint32_t status = ld_relaxed(&sy->status);
+                const uint32_t mism = ld_relaxed(&sy->mismatches);
+                const uint64_t t0 = __hip_atomic_load(&sy->t_start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t t1 = wall_clock64();
+                st_sys(&E->r.status, status);
+                st_sys(&E->r.mismatches, mism);
+                st_sys(reinterpret_cast<uint32_t *>(&E->r.t_start), (uint32_t)t0);
+                st_sys(reinterpret_cast<uint32_t *>(&E->r.t_start) + 1, (uint32_t)(t0 >> 32));
+                st_sys(reinterpret_cast<uint32_t *>(&E->r.t_end), (uint32_t)t1);
+                st_sys(reinterpret_cast<uint32_t *>(&E->r.t_end) + 1, (uint32_t)(t1 >> 32));
+                // back to zero for the entry's next request (atomic stores: the
+                // next request's parts update these words with atomics)
+                st_relaxed(&sy->arrive, 0u);
+                st_relaxed(&sy->finished, 0u);
+                st_relaxed(&sy->mismatches, 0u);
+                st_relaxed(&sy->status, 0u);
+                __hip_atomic_store(&sy->t_start, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                st_sys_release(&E->done, 1u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // dist rows of all queries (INT_MAX, 0 at the source), the query counter and
 // the launch-wide error/stats words.
 __global__ void batch_reset_kernel(uint32_t *dist_all, uint32_t n, uint32_t nq,
@@ -1180,6 +1444,50 @@ hipError_t launch_serve_verify(const ServeLaunch &S, hipStream_t s) {
     if (chunks > 256) chunks = 256;
     hipLaunchKernelGGL(serve_verify_kernel, dim3(chunks, S.nreq), dim3(256), 0, s,
                        static_cast<ServeRequest *>(S.reqs));
+    return hipGetLastError();
+}
+
+size_t stream_entry_bytes() { return sizeof(StreamEntry); }
+size_t stream_sync_bytes() { return sizeof(StreamSync); }
+
+void stream_entry_write(void *entry, const SolverLaunch &L, uint32_t src, bool fresh, const uint32_t *expect,
+                        uint32_t first, uint32_t parts, uint32_t seq) {
+    StreamEntry &e = *static_cast<StreamEntry *>(entry);
+    serve_request_write(&e.r, 0, L, src, fresh, expect, 0, parts);
+    e.first = first;
+    e.parts = parts;
+    e.seq = seq;
+    e.done = 0;
+}
+
+void stream_entry_read(const void *entry, uint32_t *done, uint32_t *status, uint32_t *mismatches,
+                       uint64_t *t_start, uint64_t *t_end) {
+    const volatile StreamEntry &e = *static_cast<const volatile StreamEntry *>(entry);
+    *done = e.done;
+    if (*done) {
+        std::atomic_thread_fence(std::memory_order_acquire);
+        *status = e.r.status;
+        *mismatches = e.r.mismatches;
+        *t_start = e.r.t_start;
+        *t_end = e.r.t_end;
+    }
+}
+
+hipError_t stream_occupancy(int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, reinterpret_cast<const void *>(&sssp_stream_kernel<1>), kWave, 0);
+}
+
+hipError_t launch_stream(const StreamLaunch &S, hipStream_t s) {
+    StreamArgs a;
+    a.host = static_cast<StreamHostCtl *>(S.host_ctl);
+    a.entries = static_cast<StreamEntry *>(S.entries);
+    a.ring = S.ring;
+    a.ring_size = S.ring_size;
+    a.dev = S.dev;
+    a.sync = static_cast<StreamSync *>(S.sync);
+    a.lifetime_ticks = S.lifetime_ticks;
+    hipLaunchKernelGGL((sssp_stream_kernel<1>), dim3(S.grid), dim3(kWave), 0, s, a);
     return hipGetLastError();
 }
 

@@ -146,14 +146,14 @@ WORKLOADS = {
     # 264,346 points, 3-NN, weight = max(1, round(len * 1e5)), source 0, delta 200000
     "road264k": dict(kind="knn", n=264_346, k=3, seed=1, weight_scale=1e5, morton=False,
                      source="zero", delta=200_000, delta_sweep=(200_000,),
-                     bench_point=dict(delta=200_000, buckets=2, bucket_window=2, l2_batch=16,
-                                      min_read=16, l1_pop=32, work_blocks=256, stale_check=2)),
+                     bench_point=dict(delta=200_000, buckets=4, bucket_window=4, l2_batch=8,
+                                      min_read=8, l1_pop=64, work_blocks=256, stale_check=3)),
     # 6,262,104 points, 3-NN, Morton order, longest edge -> 100,000, centroid source
     "road6m": dict(kind="knn", n=6_262_104, k=3, seed=2, weight_scale=-100_000.0, morton=True,
                    source="centroid", delta=200_000,
                    delta_sweep=(50_000, 100_000, 200_000, 400_000),
-                   bench_point=dict(delta=400_000, buckets=2, bucket_window=2, l2_batch=16,
-                                    min_read=16, l1_pop=32, work_blocks=384, stale_check=2)),
+                   bench_point=dict(delta=100_000, buckets=2, bucket_window=2, l2_batch=16,
+                                    min_read=16, l1_pop=32, work_blocks=256, stale_check=3)),
     # RMAT scale 22, edge factor 16, (0.57, 0.19, 0.19, 0.05), weights [1, 1000]
     "rmat22": dict(kind="rmat", scale=22, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=3,
                    wmin=1, wmax=1000, source="maxdeg", delta=200_000,
@@ -161,9 +161,9 @@ WORKLOADS = {
                    tune_grid=[dict(light_split=ls, work_blocks=wb, buckets=64 if ls else 2,
                                    bucket_window=2, l2_batch=b, min_read=b, l1_pop=b)
                               for ls in (1, 0) for wb in (256, 512) for b in (32, 64)],
-                   bench_point=dict(delta=50, light_split=1, buckets=64, bucket_window=2,
+                   bench_point=dict(delta=100, light_split=1, buckets=64, bucket_window=2,
                                     l2_batch=64, min_read=64, l1_pop=64, work_blocks=256,
-                                    stale_check=2)),
+                                    stale_check=3)),
     # RMAT scale 20 (1,048,576 vertices), edge factor 16, weights [1, 1000] (serving mix)
     "rmat20": dict(kind="rmat", scale=20, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=5,
                    wmin=1, wmax=1000, source="maxdeg", delta=200_000,

@@ -1,10 +1,12 @@
 """Python face of the serving loop (include/mlmq_serve.h).
 
 ``Server`` keeps several graphs resident, admits requests by the HBM bytes of
-their slot (distance row + queue partition) and replays an arrival trace in
-real time inside ``libmlmq.so``; ``poisson_trace`` builds such a trace.  No
-serving logic lives here: the admission, batching and launch loop is the C++
-code of ``mlmq_server_run``.
+their slot (distance row + queue partition) and either replays an arrival
+trace in real time inside ``libmlmq.so`` (``run``; ``poisson_trace`` builds
+such a trace) or serves requests as they are submitted (``submit`` /
+``poll`` / ``wait``: one persistent launch fed through a request ring).  No
+serving logic lives here: the admission, batching and launch loops are the
+C++ code of ``mlmq_server_run`` and the live service thread.
 """
 from __future__ import annotations
 
@@ -12,10 +14,12 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (MLMQError, ServeGraphParams, ServeRequest, ServeResult, ServeStats,
-                   ServerConfig, load)
+from ._lib import (MLMQ_E_PENDING, MLMQError, SERVE_MODES, ServeGraphParams, ServeRequest,
+                   ServeResult, ServeStats, ServerConfig, load)
 
 STATUS = {0: "ok", -4: "nomem", -6: "overflow", -7: "timeout", -8: "internal", -10: "rejected"}
+RESULT_FIELDS = ("latency_ms", "queue_ms", "device_ms", "status", "mismatches", "launch", "batch_size",
+                 "retries")
 
 
 def _check(rc: int, h) -> None:
@@ -44,6 +48,8 @@ class Server:
         for k, v in config.items():
             if not hasattr(cfg, k) or k == "reserved":
                 raise KeyError(f"unknown server setting {k}")
+            if k == "mode" and isinstance(v, str):
+                v = SERVE_MODES[v]
             setattr(cfg, k, v)
         self.config = {k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "reserved"}
         h = ctypes.c_void_p()
@@ -126,6 +132,52 @@ class Server:
                        ("batch_size", "i4"), ("retries", "i4"), ("reserved", "i4")])
         rec = np.frombuffer(bytes(res), dtype=dt, count=n).copy() if n else np.zeros(0, dt)
         return rec, st.as_dict()
+
+    # -- live requests ----------------------------------------------------
+    def submit(self, graph: int, src: int, dist_out=None) -> int:
+        """Submit one request now; returns its id.  `dist_out` (int32, the
+        graph's vertex count) receives the row; keep it alive until the
+        request is collected."""
+        if dist_out is not None and (dist_out.dtype != np.int32 or not dist_out.flags.c_contiguous
+                                     or len(dist_out) != self.sizes[graph][0]):
+            raise ValueError("dist_out must be contiguous int32 of the graph's size")
+        rid = ctypes.c_int64()
+        _check(self._L.mlmq_server_submit(self._h, int(graph), int(src),
+                                          None if dist_out is None else dist_out.ctypes.data,
+                                          ctypes.byref(rid)), self._h)
+        return rid.value
+
+    @staticmethod
+    def _result(r: ServeResult) -> dict:
+        return {k: getattr(r, k) for k in RESULT_FIELDS}
+
+    def poll(self, rid: int):
+        """The request's result (dict) once done, else None."""
+        r = ServeResult()
+        rc = self._L.mlmq_server_poll(self._h, int(rid), ctypes.byref(r))
+        if rc == MLMQ_E_PENDING:
+            return None
+        _check(rc, self._h)
+        return self._result(r)
+
+    def wait(self, rid: int, timeout_s: float = 0.0):
+        """Waits for the request (timeout_s <= 0: no limit); None on timeout."""
+        r = ServeResult()
+        rc = self._L.mlmq_server_wait(self._h, int(rid), float(timeout_s), ctypes.byref(r))
+        if rc == MLMQ_E_PENDING:
+            return None
+        _check(rc, self._h)
+        return self._result(r)
+
+    def drain(self, timeout_s: float = 0.0) -> bool:
+        rc = self._L.mlmq_server_drain(self._h, float(timeout_s))
+        if rc == MLMQ_E_PENDING:
+            return False
+        _check(rc, self._h)
+        return True
+
+    def stop(self) -> None:
+        _check(self._L.mlmq_server_stop(self._h), self._h)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -90,7 +90,8 @@ def test_ctypes_layouts_match_c(tmp_path):
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),
         "mlmq_queue_counts": (_lib.QueueCounts, ["capacity", "writes", "reads", "completed", "wrapped",
                                                  "reserved"]),
-        "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "spread_waves", "reserved"]),
+        "mlmq_server_config": (_lib.ServerConfig, ["timeout_s", "spread_waves", "mode", "live_waves",
+                                                   "live_idle_ms", "reserved"]),
         "mlmq_serve_graph_params": (_lib.ServeGraphParams, ["bucket_capacity", "hub_split", "light_split",
                                                               "prefetch", "stale_check", "reserved"]),
         "mlmq_serve_request": (_lib.ServeRequest, ["src", "dist_out"]),
