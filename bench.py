@@ -135,19 +135,27 @@ def main():
     ap.add_argument("--mode", choices=["single", "batch", "serve"], default="single",
                     help="single: one query per step (default); batch: all sources in one launch")
     ap.add_argument("--queries", type=int, default=256, help="batch mode: number of sources")
-    ap.add_argument("--rates", default="100,250,500,1000,2000,3000,4000,5000",
+    ap.add_argument("--rates", default="100,500,1000,1500,2000,2500,3000",
                     help="serve mode: Poisson arrival rates (queries/s)")
-    ap.add_argument("--duration", type=float, default=10.0, help="serve mode: seconds per rate")
+    ap.add_argument("--duration", type=float, default=5.0, help="serve mode: seconds per rate")
     ap.add_argument("--serve-graphs", default="road264k,road6m,rmat20")
     ap.add_argument("--mix", default="0.5,0.3,0.2", help="serve mode: graph choice weights")
-    ap.add_argument("--pool", type=int, default=16, help="serve mode: verified sources per graph")
+    ap.add_argument("--pool", type=int, default=128,
+                    help="serve mode: verified sources per graph (drawn uniformly)")
+    ap.add_argument("--serve-parts", default=None,
+                    help="serve mode: one-wave workgroups per request of each graph (comma list, "
+                         "as many as --serve-graphs); default: each graph's own setting")
+    ap.add_argument("--serve-mode", choices=["live", "lanes"], default="live",
+                    help="serve mode: live = one persistent launch fed through a request ring; "
+                         "lanes = launches of waiting requests on launch lanes")
     ap.add_argument("--budget-gib", type=float, default=64.0)
     ap.add_argument("--streams", type=int, default=4)
-    ap.add_argument("--spread-waves", type=int, default=512,
-                    help="serve mode: waves a launch of few requests may spread over")
+    ap.add_argument("--spread-waves", type=int, default=0,
+                    help="serve mode: waves a request arriving at a quiet server may spread over "
+                         "(0: each graph's own workgroup count, the lowest p99 under load)")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--max-pending", type=int, default=4096)
-    ap.add_argument("--p99-bound-ms", type=float, default=1000.0,
+    ap.add_argument("--p99-bound-ms", type=float, default=50.0,
                     help="serve mode: value = best completed rate with p99 under this bound")
     ap.add_argument("--tune", type=int, default=1, help="batch/serve modes: run the autotuner")
     ap.add_argument("--tune-single", type=int, default=0,
@@ -528,10 +536,12 @@ def run_batch(args):
 
 def run_serve(args):
     """Serving mode: Poisson arrivals over three resident graphs (default
-    road264k / road6m / rmat20 chosen 50/30/20 %), sources drawn from a seeded
-    pool per graph whose oracle rows are registered for on-device
-    verification; one trace per rate in `--rates`.  Comparator: the same
-    mix served one query at a time by plain solves."""
+    road264k / road6m / rmat20 chosen 50/30/20 %), sources drawn uniformly
+    from a seeded pool per graph (128 by default) whose oracle rows are
+    registered for on-device verification; one trace per rate in `--rates`,
+    served live (one persistent launch fed through a request ring, default)
+    or by launch lanes.  Comparator: the same mix served one query at a time
+    by plain solves."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     from mlmq_tpds_withl3_amd import Solver, make_workload, default_params
@@ -548,7 +558,7 @@ def run_serve(args):
     server = Server(0, hbm_budget=int(args.budget_gib * (1 << 30)), streams=args.streams,
                     spread_waves=args.spread_waves,
                     max_batch=args.max_batch, max_pending=args.max_pending, verify=1,
-                    timeout_s=60.0)
+                    timeout_s=60.0, mode=args.serve_mode)
     graphs, pools, seq_ms, tuned, cpu_ms = [], [], [], [], []
     P = min(16, os.cpu_count() or 4)
     ex = ThreadPoolExecutor(max_workers=P)
@@ -571,7 +581,8 @@ def run_serve(args):
         with Solver(0) as sv:
             sv.load_csr(row, col, w)
             from mlmq_tpds_withl3_amd.autotune import tune_single
-            single = tune_single(sv, pool[0], spec["delta_sweep"], grid=spec.get("tune_grid"))["params"]
+            single = dict(spec.get("bench_point") or
+                          tune_single(sv, pool[:3], spec["delta_sweep"], grid=spec.get("tune_grid"))["params"])
             sp = default_params(timeout_s=60.0, **single)
             sv.solve(pool[0], sp)
             t1 = time.perf_counter()
@@ -581,6 +592,11 @@ def run_serve(args):
         serve_keys = ("delta", "buckets", "bucket_window", "l2_batch", "min_read", "l1_pop",
                       "light_split")
         base = {k: v for k, v in single.items() if k in serve_keys}
+        if args.serve_parts:
+            parts = int(args.serve_parts.split(",")[gi])
+            base.update(workgroups=max(1, parts // 8), waves=8 if parts >= 8 else 1)
+            if parts < 8:
+                base["workgroups"] = parts
         if args.tune:
             t = tune_serving(server, gid, pool, base=base, repeats=2, log=log)
             tuned.append(t["params"])
@@ -630,10 +646,11 @@ def run_serve(args):
         "steps": len(rates), "warmup": 0, "ms_per_step": round(args.duration * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "uint32",
         "data": "synthetic",
-        "config": {"workload": "serve:" + "+".join(names), "mode": "serve", "mix": mix,
+        "config": {"workload": "serve:" + "+".join(names), "mode": "serve",
+                   "serve_mode": args.serve_mode, "mix": mix,
                    "duration_s": args.duration, "pool_per_graph": args.pool,
                    "budget_gib": args.budget_gib, "streams": args.streams,
-                   "spread_waves": args.spread_waves,
+                   "spread_waves": args.spread_waves, "serve_parts": args.serve_parts,
                    "max_batch": args.max_batch, "max_pending": args.max_pending,
                    "p99_bound_ms": args.p99_bound_ms, "graphs": graphs},
         "best_rate": best["rate"] if best else None,

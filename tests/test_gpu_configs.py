@@ -11,9 +11,9 @@ Every distance of every query is compared with the CPU oracle:
   launch, every row checked (the reference's per-source check,
   SSSP/main.cu:195-209);
 * a Poisson serving run over road264k, road6m and RMAT-20 chosen 50/30/20 %,
-  with uniformly random sources: every served row is compared on the device
-  with its oracle row, and every 8th row is also copied out and compared on
-  the host.
+  with uniformly random sources, by launch lanes and by the live persistent
+  launch: every served row is compared on the device with its oracle row,
+  and every 8th row is also copied out and compared on the host.
 
 Oracle rows are computed on a thread pool (the oracle is C; ctypes releases
 the GIL).
@@ -120,7 +120,8 @@ def test_road1m_256_queries_one_launch(solver):
     g.close()
 
 
-def test_serving_mix_random_sources():
+@pytest.mark.parametrize("mode", ["lanes", "live"])
+def test_serving_mix_random_sources(mode):
     """Poisson arrivals (400 queries/s for 3 s) over three resident graphs at
     50/30/20 %, sources uniform over each graph's vertices; every result is
     verified."""
@@ -137,7 +138,8 @@ def test_serving_mix_random_sources():
     arrival = np.cumsum(gaps)
     arrival = arrival[arrival < duration]
     gsel = rng.choice(3, size=len(arrival), p=[0.5, 0.3, 0.2]).astype(np.int32)
-    server = Server(0, hbm_budget=48 << 30, streams=4, max_batch=64, timeout_s=60.0, verify=1)
+    server = Server(0, hbm_budget=48 << 30, streams=4, max_batch=64, timeout_s=60.0, verify=1,
+                    mode=mode)
     graphs, srcs = [], np.empty(len(arrival), dtype=np.int32)
     for gi, name in enumerate(names):
         g, _, _ = make_workload(name)
@@ -158,7 +160,7 @@ def test_serving_mix_random_sources():
     outs = [np.empty(server.sizes[gsel[i]][0], dtype=np.int32) if i % 8 == 0 else None
             for i in range(len(arrival))]
     res, st = server.run(arrival, gsel, srcs, dist_out=outs)
-    say(f"serve: {st['completed']} completed, {st['verified']} verified, "
+    say(f"serve ({mode}): {st['completed']} completed, {st['verified']} verified, "
         f"{st['completed_qps']:.1f} q/s, p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, "
         f"mean batch {st['mean_batch']:.1f}")
     assert st["completed"] == len(arrival) and st["failed"] == 0 and st["rejected"] == 0
