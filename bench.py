@@ -259,9 +259,15 @@ def main():
             # the vertices), so the timed point is not fitted to the measured
             # query: the workload's delta sweep crossed with its grid.
             from mlmq_tpds_withl3_amd.autotune import tune_single
+            # Tuning sources: seeded, uniform over the vertices the timed source
+            # reaches (its component; on power-law graphs many vertices are
+            # isolated and a query from them is trivial), other than itself.
+            solver.solve(src, make_params(point.get("delta", spec["delta"])))
+            reach = np.flatnonzero(solver.dist() != 0x7FFFFFFF)
+            reach = reach[reach != src]
             rng = np.random.default_rng(args.source_seed)
-            tsrc = [int(x) for x in rng.integers(0, g.n, size=args.tune_sources + 1) if x != src]
-            tsrc = tsrc[:args.tune_sources]
+            tsrc = [int(x) for x in rng.choice(reach, size=min(args.tune_sources, len(reach)),
+                                                replace=False)]
             tuned = tune_single(solver, tsrc, spec["delta_sweep"], grid=spec.get("tune_grid"),
                                 log=log if args.verbose else None)
             point = tuned["params"]
