@@ -448,10 +448,12 @@ def run_batch(args):
     row, col, w = g.arrays()
     sources = np.random.default_rng(args.source_seed).integers(0, g.n, size=args.queries)
     delta = args.delta if args.delta is not None else spec["delta"]
+    # timed (and tuned) without the statistics counters, which cost registers
+    # and so waves per CU; one more launch with them supplies the work counters
     base = dict(delta=delta, batch_waves=args.batch_waves, work_blocks=args.work_blocks,
                 l1_pop=args.l1_pop, min_read=args.min_read, l2_batch=args.l2_batch,
                 buckets=args.buckets, bucket_window=args.window, prefetch=args.prefetch,
-                stale_check=args.stale_check)
+                stale_check=args.stale_check, batch_stats=0)
     solver = Solver(0)
     solver.load_csr(row, col, w)
     tuned = None
@@ -469,6 +471,7 @@ def run_batch(args):
         times.append(solver.solve_batch(sources, params))
     wall = time.perf_counter() - wall0
     log(f"batch: {args.steps} x {args.queries} queries in {wall * 1e3:.1f} ms")
+    counted = solver.solve_batch(sources, default_params(timeout_s=60.0, **dict(base, batch_stats=1)))
     # verify every query against the oracle (thread pool; ctypes releases the
     # GIL); the same pass, timed, is the CPU baseline: P independent Dijkstras
     pool = min(16, os.cpu_count() or 4)
@@ -495,6 +498,7 @@ def run_batch(args):
     seq_qps = args.queries / seq_wall
     last = times[-1]
     dev_avg_s = statistics.mean(t["device_ms"] for t in times) / 1e3
+    work = counted   # the statistics launch (same queries and setting)
     achieved = b_alg / dev_avg_s / 1e9
     out = {
         "metric": METRIC, "value": round(qps, 3), "unit": "queries/s", "n_gpus": 1,
@@ -514,11 +518,11 @@ def run_batch(args):
                        "solve_ms_median": round(statistics.median(seq_ms), 3)},
         "speedup_vs_sequential": round(qps / seq_qps, 3),
         "device_ms": {"mean": round(dev_avg_s * 1e3, 4)},
-        "work": {"relaxations": last["relaxations"], "improvements": last["improvements"],
-                 "l2_pushes": last["l2_pushes"], "stale": last["stale"],
-                 "batches": last["batches"], "idle_polls": last["idle_polls"],
-                 "reachable_edges": e_r,
-                 "inflation": round(last["relaxations"] / max(1, e_r), 4)},
+        "work": {"relaxations": work["relaxations"], "improvements": work["improvements"],
+                 "l2_pushes": work["l2_pushes"], "stale": work["stale"],
+                 "batches": work["batches"], "idle_polls": work["idle_polls"],
+                 "reachable_edges": e_r, "device_ms_with_counters": round(work["device_ms"], 4),
+                 "inflation": round(work["relaxations"] / max(1, e_r), 4)},
         "correct": correct,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -69,7 +69,8 @@ typedef struct mlmq_params {
     int32_t  work_blocks;     /* persistent workgroups of one wave64 each; 0 = automatic
                                  (blocks_per_cu x CUs if blocks_per_cu > 0, else n / 25000
                                  clamped to [128, 512]); mlmq_solve_batch: workgroups of
-                                 batch_waves waves, one query each */
+                                 batch_waves waves, one query each (0 = as many as the device
+                                 holds at once, at most the number of queries) */
     int32_t  blocks_per_cu;   /* used when work_blocks == 0 */
     int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
     int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */
@@ -125,7 +126,11 @@ typedef struct mlmq_params {
                                  0 or 512 (default); mlmq_solve_batch also takes 1024, which lets
                                  l1_spill go up to 255 (the spill threshold must leave room for one
                                  batch's pushes: l1_spill <= l1_ring - 448) */
-    int32_t  reserved[3];
+    int32_t  batch_stats;     /* mlmq_solve_batch: 1 (default) counts relaxations, improvements,
+                                 pushes, stale records, batches and idle polls into
+                                 mlmq_batch_timing; 0 leaves them out (zero there): the counters
+                                 then need no registers and a CU holds more waves */
+    int32_t  reserved[2];
 } mlmq_params;
 
 /* Per-solve statistics (all counts summed over wavefronts). */

@@ -71,7 +71,8 @@ class Params(ctypes.Structure):
         ("l1_blocks", ctypes.c_int32), ("idle_backoff", ctypes.c_int32),
         ("defer_heavy", ctypes.c_int32), ("strict_window", ctypes.c_int32),
         ("light_split", ctypes.c_int32), ("csr_stream", ctypes.c_int32),
-        ("l1_ring", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("l1_ring", ctypes.c_int32), ("batch_stats", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -181,7 +181,8 @@ def tune_batch(solver, sources, base: dict | None = None, log=None) -> dict:
     # The L1 ring (the LDS segment of each wave) and its spill threshold: a
     # 1024-record ring lets a wave keep up to 255 records before spilling to
     # L2 (a 512-record ring at most 64).
-    space = [("batch_waves", [4, 8, 16]), ("delta", None),
+    space = [("batch_waves", [4, 8, 16]), ("l1_ring", [512, 1024]), ("batch_waves", [4, 8, 16]),
+             ("delta", None),
              ("l1_pop", [16, 32, 64]), ("buckets", [2, 4, 8, 16, 32]), ("min_read", [8, 16, 32]),
              ("l1_ring", [512, 1024]), ("l1_spill", [64, 128, 255]),
              ("prefetch", [0, 1]), ("stale_check", [1, 3, 2]),

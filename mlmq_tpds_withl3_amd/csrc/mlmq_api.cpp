@@ -410,6 +410,7 @@ int check_params(const mlmq_params &p, bool batch, std::string *why) {
     if (p.l1_blocks != 0 && p.l1_blocks != 1) return bad(MLMQ_E_INVALID, "l1_blocks must be 0 or 1");
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return bad(MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.csr_stream != 0 && p.csr_stream != 1) return bad(MLMQ_E_INVALID, "csr_stream must be 0 or 1");
+    if (p.batch_stats != 0 && p.batch_stats != 1) return bad(MLMQ_E_INVALID, "batch_stats must be 0 or 1");
     if (p.light_split && (p.hub_split || p.defer_heavy))
         return bad(MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
     if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
@@ -664,6 +665,7 @@ void mlmq_params_default(mlmq_params *p) {
     p->light_split = 0;
     p->csr_stream = 0;
     p->l1_ring = 512;
+    p->batch_stats = 1;
 }
 
 int mlmq_create(int device, mlmq_ctx **out) {
@@ -929,9 +931,10 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
 
     int occ = 0;
     const int l1_cap = p.l1_ring ? p.l1_ring : 512;
-    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, l1_cap, &occ));
+    const bool ext = p.light_split || p.defer_heavy || p.strict_window;
+    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, l1_cap, ext, p.batch_stats != 0, &occ));
     if (occ < 1) return fail(c, MLMQ_E_HIP, "batch kernel cannot be resident");
-    int blocks = p.work_blocks ? p.work_blocks : c->cus;
+    int blocks = p.work_blocks ? p.work_blocks : occ * c->cus;
     if (blocks > occ * c->cus) blocks = occ * c->cus;
     if (blocks > nq) blocks = nq;
 
@@ -1038,6 +1041,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         B.qtime = c->b_qtime;
         B.waves = p.batch_waves;
         B.l1_cap = l1_cap;
+        B.stats = p.batch_stats != 0;
 
         // The launch (row reset + batch kernel) is captured once per shape and
         // replayed as a hipGraph; the source list is copied in before it.
@@ -1051,7 +1055,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check,
             (uint64_t)L.prefetch, (uint64_t)L.max_sleep, (uint64_t)L.defer_heavy,
             (uint64_t)L.strict_window, (uint64_t)L.light_split, (uint64_t)L.csr_stream,
-            (uint64_t)l1_cap};
+            (uint64_t)l1_cap, (uint64_t)B.stats};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

@@ -86,6 +86,7 @@ struct BatchLaunch {
     uint64_t *qtime;
     int waves;   // waves per workgroup (= per query): 4, 8 or 16
     int l1_cap;  // L1 ring records per wave: 512 or 1024
+    bool stats;  // count relaxations etc. (more registers per wave)
 };
 
 // Serving launches: a table of requests, each with its own graph, queue
@@ -136,7 +137,7 @@ hipError_t stream_occupancy(int *blocks_per_cu);
 hipError_t launch_stream(const StreamLaunch &S, hipStream_t s);
 hipError_t launch_serve(const ServeLaunch &S, hipStream_t s);
 hipError_t launch_serve_verify(const ServeLaunch &S, hipStream_t s);
-hipError_t batch_occupancy(int waves, int l1_cap, int *blocks_per_cu);
+hipError_t batch_occupancy(int waves, int l1_cap, bool ext, bool stats, int *blocks_per_cu);
 hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
                               uint32_t *gctl, hipStream_t s);
 hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s);

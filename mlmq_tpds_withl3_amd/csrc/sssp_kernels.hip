@@ -552,12 +552,11 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
     // work cannot let the solve end anyway, and returning them whenever L1
     // drains mid-run put a non-returning atomic in flight that the next
     // batch's loads waited behind (vmcnt counts in issue order).  They are
-    // returned with a relaxed atomic add: termination needs every tail
-    // reservation a record's expansion made to be performed before its
-    // credit, and each reservation's result (the slot index) was waited for
-    // before the record's children were written, so program order already
-    // gives that.  A release would also write back this XCD's L2 and wait for
-    // the in-flight prefetch loads.
+    // returned with a release: termination needs every tail reservation a
+    // record's expansion made (and the children written into the slots) to
+    // be visible before its credit, which the quiescence check reads with an
+    // acquire.  The release writes back this XCD's L2 and waits for the
+    // wave's loads in flight, but the wave is idle here.
     uint32_t credit = 0;
     uint32_t cl_start = 0;      // lane b: start of the claim on bucket b
     uint64_t cl_mask = 0;       // lane b: pending slots of that claim
@@ -596,7 +595,8 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         if (nin == 0) {
             if (credit) {
-                if (lane == 0) fetch_add(q.ctl + kDoneOff, credit);
+                if (lane == 0)
+                    __hip_atomic_fetch_add(q.ctl + kDoneOff, credit, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 credit = 0;
             }
             if (a.donate_min() && !idle_listed) {
@@ -706,7 +706,10 @@ struct BatchArgs {
     uint64_t *qtime;           // per query: wall-clock ticks at start and end
 };
 
-template <int kWaves, bool kProf, bool kExt = false, uint32_t kCap = kL1Cap>
+// kStats = false leaves the statistics out: the counters then need no
+// registers (the per-wave counters otherwise spill into VGPR lanes and bound
+// the waves per SIMD at 4; without them it is 5).
+template <int kWaves, bool kProf, bool kExt = false, uint32_t kCap = kL1Cap, bool kStats = true>
 __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
     __shared__ uint64_t s_ring[kWaves][kCap + 64];
     __shared__ uint32_t s_map[kWaves][kWave];
@@ -762,7 +765,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
         }
         __syncthreads();
     }
-    flush_stats(b.gctl, st);
+    if (kStats) flush_stats(b.gctl, st);
 }
 
 // ---- serving: requests of any resident graphs, several workgroups each ----
@@ -1302,14 +1305,19 @@ hipError_t solver_occupancy(bool slf, bool hot, int *blocks_per_cu) {
                                                         kWavesPerBlock * kWave, 0);
 }
 
-template <int kW>
-static const void *batch_fn(int cap) {
-    return cap == 1024 ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, false, 1024>)
-                       : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, false, kL1Cap>);
+template <int kW, bool kExt>
+static const void *batch_fn(int cap, bool stats) {
+    if (cap == 1024)
+        return stats ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, 1024, true>)
+                     : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, 1024, false>);
+    return stats ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, kL1Cap, true>)
+                 : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, kL1Cap, false>);
 }
 
-hipError_t batch_occupancy(int waves, int l1_cap, int *blocks_per_cu) {
-    const void *fn = waves == 16 ? batch_fn<16>(l1_cap) : waves == 8 ? batch_fn<8>(l1_cap) : batch_fn<4>(l1_cap);
+hipError_t batch_occupancy(int waves, int l1_cap, bool ext, bool stats, int *blocks_per_cu) {
+    const void *fn = waves == 16 ? (ext ? batch_fn<16, true>(l1_cap, stats) : batch_fn<16, false>(l1_cap, stats))
+                   : waves == 8  ? (ext ? batch_fn<8, true>(l1_cap, stats) : batch_fn<8, false>(l1_cap, stats))
+                                 : (ext ? batch_fn<4, true>(l1_cap, stats) : batch_fn<4, false>(l1_cap, stats));
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
 }
 
@@ -1350,20 +1358,24 @@ hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const
 }
 
 template <int kW, bool kExt, uint32_t kCap>
-static void launch_batch_instance(const dim3 &grid, const BatchArgs &b, hipStream_t s) {
-    hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap>), grid, dim3(kW * kWave), 0, s, b);
+static void launch_batch_instance(const dim3 &grid, const BatchArgs &b, bool stats, hipStream_t s) {
+    if (stats)
+        hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap, true>), grid, dim3(kW * kWave), 0, s, b);
+    else
+        hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap, false>), grid, dim3(kW * kWave), 0, s, b);
 }
 
 template <int kW, bool kExt>
-static void launch_batch_cap(const dim3 &grid, const BatchArgs &b, int cap, hipStream_t s) {
-    if (cap == 1024) launch_batch_instance<kW, kExt, 1024>(grid, b, s);
-    else launch_batch_instance<kW, kExt, kL1Cap>(grid, b, s);
+static void launch_batch_cap(const dim3 &grid, const BatchArgs &b, int cap, bool stats, hipStream_t s) {
+    if (cap == 1024) launch_batch_instance<kW, kExt, 1024>(grid, b, stats, s);
+    else launch_batch_instance<kW, kExt, kL1Cap>(grid, b, stats, s);
 }
 
 template <int kW>
-static void launch_batch_ext(const dim3 &grid, const BatchArgs &b, bool ext, int cap, hipStream_t s) {
-    if (ext) launch_batch_cap<kW, true>(grid, b, cap, s);
-    else launch_batch_cap<kW, false>(grid, b, cap, s);
+static void launch_batch_ext(const dim3 &grid, const BatchArgs &b, bool ext, int cap, bool stats,
+                             hipStream_t s) {
+    if (ext) launch_batch_cap<kW, true>(grid, b, cap, stats, s);
+    else launch_batch_cap<kW, false>(grid, b, cap, stats, s);
 }
 
 hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s) {
@@ -1379,9 +1391,9 @@ hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t
     b.qtime = B.qtime;
     const dim3 grid(L.grid_blocks);
     const bool ext = L.light_split || L.defer_heavy || L.strict_window;
-    if (B.waves == 16) launch_batch_ext<16>(grid, b, ext, B.l1_cap, s);
-    else if (B.waves == 8) launch_batch_ext<8>(grid, b, ext, B.l1_cap, s);
-    else launch_batch_ext<4>(grid, b, ext, B.l1_cap, s);
+    if (B.waves == 16) launch_batch_ext<16>(grid, b, ext, B.l1_cap, B.stats, s);
+    else if (B.waves == 8) launch_batch_ext<8>(grid, b, ext, B.l1_cap, B.stats, s);
+    else launch_batch_ext<4>(grid, b, ext, B.l1_cap, B.stats, s);
     return hipGetLastError();
 }
 

@@ -84,7 +84,8 @@ def test_ctypes_layouts_match_c(tmp_path):
     structs = {
         "mlmq_params": (_lib.Params, ["delta", "bucket_capacity", "timeout_s", "use_graph", "hub_split",
                                        "stale_check", "prefetch", "l1_blocks", "idle_backoff", "defer_heavy",
-                                       "strict_window", "light_split", "csr_stream", "l1_ring", "reserved"]),
+                                       "strict_window", "light_split", "csr_stream", "l1_ring", "batch_stats",
+                                       "reserved"]),
         "mlmq_timing": (_lib.Timing, ["donated", "hub_chunks"]),
         "mlmq_batch_timing": (_lib.BatchTiming, None),
         "mlmq_graph_info": (_lib.GraphInfo, ["diameter"]),

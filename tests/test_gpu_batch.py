@@ -167,3 +167,21 @@ def test_batch_errors(solver):
         solver.solve_batch(list(range(16)), timeout_s=1e-7, delta=10)
     assert e.value.code == -7
     check_batch(solver, row, col, w, list(range(0, 10_000, 700)))
+
+
+@pytest.mark.parametrize("waves", [4, 8, 16])
+def test_batch_without_counters_fills_the_device(solver, waves):
+    """batch_stats 0: the instance without statistics counters (more waves
+    per CU) gives the same rows; work_blocks 0 runs as many workgroups as
+    the device holds, at most one per query."""
+    say(f"batch without counters, {waves} waves")
+    row, col, w = grid_csr(50, 60, seed=51)
+    solver.load_csr(row, col, w)
+    rng = np.random.default_rng(52)
+    sources = [int(x) for x in rng.integers(0, 3000, size=700)]
+    t = check_batch(solver, row, col, w, sources, batch_waves=waves, batch_stats=0, delta=3000)
+    assert t["relaxations"] == 0 and t["improvements"] == 0
+    t1 = check_batch(solver, row, col, w, sources[:40], batch_waves=waves, batch_stats=1, delta=3000)
+    assert t1["relaxations"] > 0 and t1["work_blocks"] == 40
+    say(f"{waves} waves: {t['work_blocks']} workgroups for {len(sources)} queries")
+    assert t["work_blocks"] >= 256
