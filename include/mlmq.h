@@ -94,11 +94,13 @@ typedef struct mlmq_params {
                                  chunk and sends the rest to L2 as chunk records that other
                                  waves relax; 0 (default): one wave relaxes the whole list */
     int32_t  stale_check;     /* filter of popped records whose distance has since dropped:
-                                 2 (default) coherent load of dist[v] with the hop block, 1 load
+                                 2 (default) coherent read of dist[v] with the hop block, 1 load
                                  through the L2 (may miss a drop made on another XCD), 3 late:
-                                 coherent load in the atomicMins' round trip (a stale record's
+                                 coherent read in the atomicMins' round trip (a stale record's
                                  inline atomics are issued, its pushes and CSR edges dropped),
-                                 0 none. Results are exact in every mode; only the work differs */
+                                 0 none. A coherent read is atomicMin(dist[v], d), which cannot
+                                 change dist[v] (<= d). Results are exact in every mode; only
+                                 the work differs */
     int32_t  prefetch;        /* 1 (default): while its atomicMin is in flight, touch each
                                  relaxed child's hop block so a later pop finds it in L2 */
     int32_t  l1_blocks;       /* mlmq_solve with an L1: 1 = each L1 record carries its hop block
