@@ -32,7 +32,8 @@ def tune_single(solver, src, deltas, grid=None, repeats: int = 2, confirm: int =
     sources: a point's time is then the mean over them): every delta in
     `deltas` crossed with `grid` (a list of parameter dicts, default: L1 pop
     size / L2 top-up threshold pairs, bucket counts and three grid sizes),
-    then the late and L2-cached stale checks at the four best points.  Each
+    then the late and L2-cached stale checks at the four best points and
+    child-block touches off at the four best after that.  Each
     point is the best device time of `repeats` solves per source; the four
     best are then timed `confirm` times more and the lowest median wins (a
     best-of-two is often a lucky draw).  Returns the best point, its median
@@ -59,11 +60,16 @@ def tune_single(solver, src, deltas, grid=None, repeats: int = 2, confirm: int =
     for d in deltas:
         for g in grid:
             measure(dict(g, delta=int(d)))
-    # second stage: the other stale-check modes at the four best points
+    # second stage: the other stale-check modes at the four best points, then
+    # child-block touches off at the four best so far (on power-law graphs the
+    # touched blocks are mostly never popped)
     for bp, _ in sorted(table, key=lambda t: t[1])[:4]:
         for sc in (3, 1):
             if bp.get("stale_check", 2) != sc:
                 measure(dict(bp, stale_check=sc))
+    for bp, _ in sorted(table, key=lambda t: t[1])[:4]:
+        if bp.get("prefetch", 1) != 0:
+            measure(dict(bp, prefetch=0))
     finals = []
     for bp, _ in sorted(table, key=lambda t: t[1])[:4]:
         p = default_params(timeout_s=30.0, **bp)

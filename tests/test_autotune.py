@@ -62,6 +62,18 @@ def test_tune_single_finds_best_and_tries_stale_modes():
     assert len(r["confirmed"]) == 4
 
 
+def test_tune_single_tries_touches_off():
+    """Touches off is measured at the best points of the second stage and
+    wins when it is faster; when it is slower the touches stay on."""
+    for off_cost, want in ((-0.3, 0), (+0.3, 1)):
+        s = FakeSolver(lambda d: single_cost(d) + (off_cost if d.get("prefetch", 1) == 0 else 0.0))
+        r = A.tune_single(s, 0, (50000, 100000, 200000), repeats=1)
+        assert r["params"].get("prefetch", 1) == want
+        assert r["params"]["stale_check"] == 3 and r["params"]["delta"] == 100000
+        assert any(d.get("prefetch", 1) == 0 and d["delta"] == 100000 and d["stale_check"] == 3
+                   for d in s.seen)
+
+
 def test_tune_single_custom_grid():
     s = FakeSolver(lambda d: 1.0 + d["light_split"] * -0.5 + d["work_blocks"] / 1e4)
     grid = [dict(light_split=ls, work_blocks=wb) for ls in (0, 1) for wb in (256, 512)]
