@@ -60,6 +60,21 @@ __device__ __forceinline__ uint32_t l1_pop(const uint64_t *ring, L1State &l1, ui
     return k;
 }
 
+// Pop for lane pairs: up to `maxn` (<= 32) records from the front, record i
+// into lanes 2i and 2i + 1.
+template <uint32_t kCap = kL1Cap>
+__device__ __forceinline__ uint32_t l1_pop_pair(const uint64_t *ring, L1State &l1, uint32_t maxn,
+                                                uint64_t &rec) {
+    l1_uniform(l1);
+    const uint32_t k = uniform(l1.size < maxn ? l1.size : maxn);
+    const uint32_t r = lane_id() >> 1;
+    if (r < k) rec = ring[(l1.head + r) & (kCap - 1)];
+    wave_sync();
+    l1.head = (l1.head + k) & (kCap - 1);
+    l1.size -= k;
+    return k;
+}
+
 // Pop for the block-carrying L1: like l1_pop, and each popped lane also gets
 // the block slot of its record.  When the head record is hot (its block is
 // in LDS) only the leading run of hot records is taken, so the batch needs

@@ -478,7 +478,7 @@ int plan_query(mlmq_ctx *c, int32_t src, const mlmq_params *params, QueryPlan *q
     HIP_TRY(c, hipSetDevice(c->device));
 
     int occ = 0;
-    HIP_TRY(c, mlmq::solver_occupancy(q->slf, q->hot, &occ));
+    HIP_TRY(c, mlmq::solver_occupancy(q->slf, q->hot, mlmq::lane_pairs_for((uint32_t)p.l1_pop), &occ));
     if (occ < 1) return fail(c, MLMQ_E_HIP, "solver kernel cannot be resident");
     int blocks = p.work_blocks;
     if (blocks == 0 && p.blocks_per_cu > 0) {
@@ -818,7 +818,7 @@ int mlmq_device_shape(mlmq_ctx *c, int32_t *blocks_per_cu_max, int32_t *cus, int
     if (!c) return MLMQ_E_INVALID;
     HIP_TRY(c, hipSetDevice(c->device));
     int b = 0;
-    HIP_TRY(c, mlmq::solver_occupancy(true, false, &b));
+    HIP_TRY(c, mlmq::solver_occupancy(true, false, false, &b));
     if (blocks_per_cu_max) *blocks_per_cu_max = b;
     if (cus) *cus = c->cus;
     if (threads) *threads = mlmq::kWavesPerBlock * mlmq::kWave;
@@ -932,7 +932,8 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
     int occ = 0;
     const int l1_cap = p.l1_ring ? p.l1_ring : 512;
     const bool ext = p.light_split || p.defer_heavy || p.strict_window;
-    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, l1_cap, ext, p.batch_stats != 0, &occ));
+    HIP_TRY(c, mlmq::batch_occupancy(p.batch_waves, l1_cap, ext, p.batch_stats != 0,
+                                     mlmq::lane_pairs_for((uint32_t)p.l1_pop), &occ));
     if (occ < 1) return fail(c, MLMQ_E_HIP, "batch kernel cannot be resident");
     int blocks = p.work_blocks ? p.work_blocks : occ * c->cus;
     if (blocks > occ * c->cus) blocks = occ * c->cus;

@@ -137,7 +137,7 @@ hipError_t stream_occupancy(int *blocks_per_cu);
 hipError_t launch_stream(const StreamLaunch &S, hipStream_t s);
 hipError_t launch_serve(const ServeLaunch &S, hipStream_t s);
 hipError_t launch_serve_verify(const ServeLaunch &S, hipStream_t s);
-hipError_t batch_occupancy(int waves, int l1_cap, bool ext, bool stats, int *blocks_per_cu);
+hipError_t batch_occupancy(int waves, int l1_cap, bool ext, bool stats, bool pair, int *blocks_per_cu);
 hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const uint32_t *sources,
                               uint32_t *gctl, hipStream_t s);
 hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s);
@@ -151,7 +151,10 @@ hipError_t launch_light_count(const uint32_t *row, const uint32_t *wgt, uint32_t
 hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
                                uint32_t n, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);
-hipError_t solver_occupancy(bool slf, bool hot, int *blocks_per_cu);
+// Lane pairs (two lanes per record, four edge slots each) whenever a batch
+// pops at most 32 records; the serving kernels always use them.
+inline bool lane_pairs_for(uint32_t pop_max) { return pop_max <= 32u; }
+hipError_t solver_occupancy(bool slf, bool hot, bool pair, int *blocks_per_cu);
 hipError_t launch_solver(const SolverLaunch &L, hipStream_t s);
 
 }  // namespace mlmq

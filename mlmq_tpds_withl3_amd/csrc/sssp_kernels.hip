@@ -313,13 +313,27 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 // which keep the whole device busy, are bound by the memory-side atomic
 // traffic and gain from it; the single-query kernel, bound by the latency
 // of its critical path, does not.
-template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre, uint32_t kCap = kL1Cap>
+// kPair: lane pairs.  Record i sits in lanes 2i and 2i + 1 (at most 32
+// records a batch); the even lane loads the first half of the hop block
+// (degree, CSR start and inline edges 0-2), the odd lane the second half
+// (edges 3-6), and each relaxes its own edges, so a batch runs 4 edge slots
+// per lane instead of 7 and fills the wave when a pop takes 32 records or
+// fewer.  The even lane speaks for the record: it issues the stale checks,
+// owns the CSR edges of heavier vertices and the heavy-phase record; its
+// results reach the odd lane by a DPP quad permutation, issued with every
+// lane active (a DPP whose source lane is masked off reads 0).
+template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre, uint32_t kCap = kL1Cap,
+          bool kPair = false>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
                               const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
                               uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &sunk,
                               uint32_t &hungry, WaveStats &st, PhaseClock<kProf> &clk) {
+    static_assert(!kPair || (kSLF && !kHot), "lane pairs need the record-only L1");
+    constexpr int kSlots = kPair ? 4 : (int)kInlineEdges;
     const uint32_t lane = lane_id();
-    const bool valid = lane < nin;
+    const uint32_t half = kPair ? (lane & 1u) : 0u;
+    const bool valid = (kPair ? (lane >> 1) : lane) < nin;
+    const bool lead = valid && half == 0;
     uint32_t v = rec_vertex(rec);
     // Light/heavy split: a heavy-phase record relaxes only its vertex's edges
     // of weight >= delta (the reference has no such split; see SolveArgs).
@@ -356,10 +370,22 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         // The block first: the compiler waits for every outstanding load at
         // the join of the stale-check modes' branches, and with the block
         // already issued that wait covers both loads in one round trip.
-        b = load_block(a.blocks, v);
-        if (a.stale_check() == 2) cur = fetch_min(a.dist + v, d);
-        else if (a.stale_check() == 1) cur = a.dist[v];
+        if (kPair) {
+            // this lane's half of the block: words 0-7 or 8-15
+            const uint4 *p = a.blocks + 4ull * v + 2u * half;
+            b.q[0] = p[0];
+            b.q[1] = p[1];
+            if (half == 0) {
+                if (a.stale_check() == 2) cur = fetch_min(a.dist + v, d);
+                else if (a.stale_check() == 1) cur = a.dist[v];
+            }
+        } else {
+            b = load_block(a.blocks, v);
+            if (a.stale_check() == 2) cur = fetch_min(a.dist + v, d);
+            else if (a.stale_check() == 1) cur = a.dist[v];
+        }
     }
+    if (kPair) cur = dpp<0xA0>(cur);   // quad_perm [0,0,2,2]: the even lane's value
     if (kHot) {
         // word-wise select of register values (keeps both sources out of memory)
         uint32_t *bw = reinterpret_cast<uint32_t *>(&b);
@@ -370,24 +396,26 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             bw[i] = hot ? h : bw[i];
         }
     }
+    const uint32_t bdeg = kPair ? dpp<0xA0>(b.q[0].x) : b.deg();
     // idle-wave count for work donation, in the same round trip
     hungry = a.donate_min() ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
     // bucket epoch, for deferring heavy records (same round trip)
     const uint32_t epoch = (kExt && a.defer_heavy()) ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
     bool keep = valid && (hot || d <= cur);
-    st.v[kStatStale] += popc(ballot(valid && !keep));
+    st.v[kStatStale] += popc(ballot((kPair ? lead : valid) && !keep));
     // Deferral: a heavy record whose distance lies beyond the head bucket goes
     // back to L2 (into its own bucket) instead of being expanded now, so hub
     // edge lists are expanded in bucket order and rarely twice.
     if (kExt && a.defer_heavy()) {
-        const bool defer = keep && !chunk && b.deg() > kInlineEdges && d / a.q.delta > epoch;
+        const bool defer = keep && !chunk && half == 0 && bdeg > kInlineEdges &&
+                           d / a.q.delta > epoch;
         if (ballot(defer)) {
             st.v[kStatSpare1] += popc(ballot(defer));
             if (!l2_push(a.q, make_rec(v, d), defer, st)) return false;
             keep = keep && !defer;
         }
     }
-    uint32_t deg = keep ? b.deg() : 0u;
+    uint32_t deg = keep ? bdeg : 0u;
     uint32_t hstart = b.start();
     uint32_t nextra = 0;   // chunks of a hub popped here that go to L2
     const uint32_t hub_word = b.q[0].z, csize = b.q[0].w;
@@ -400,7 +428,8 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         deg = min(deg, csize);
     }
     // chunk lanes always take the CSR path (their edges are not inline)
-    const bool csr_path = chunk || b.deg() > kInlineEdges;
+    const bool csr_path = chunk || bdeg > kInlineEdges;
+    if (half) nextra = 0;   // (the odd lane's header words are edges)
     bool need_heavy = false;   // a light-phase record whose vertex has heavy edges
     if ((kExt && a.light_split()) && csr_path && !chunk) {
         // adjacency sorted by weight: the first b.q[1].x edges are light
@@ -409,31 +438,41 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             hstart += nl;
             deg -= nl;
         } else {
-            need_heavy = deg > nl;
+            need_heavy = deg > nl && half == 0;
             deg = nl;
         }
     }
-    bool heavy = csr_path && deg > 0;
+    bool heavy = csr_path && deg > 0 && half == 0;
     const uint32_t ideg = csr_path ? 0u : deg;
     clk.lap(st, kClkRow);
     if (ballot(nextra > 0)) {
         if (!push_chunks(a, hub_word - 1, d, nextra, map, st)) return false;
     }
 
-    uint32_t c[kInlineEdges], nd[kInlineEdges], pv[kInlineEdges];
-    uint32_t on = 0;   // bit k: inline edge k is relaxed by this record
+    // Slot k holds edge k; with lane pairs slot k holds the lane's words 2k
+    // and 2k + 1: edge k - 1 on the even lane (slot 0 is the header), edge
+    // k + 3 on the odd lane.
+    uint32_t c[kSlots], nd[kSlots], pv[kSlots];
+    uint32_t on = 0;   // bit k: slot k's edge is relaxed by this record
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) {
-        c[k] = word_of(b, 2 + 2 * k);
-        const uint32_t wk = word_of(b, 3 + 2 * k);
+    for (int k = 0; k < kSlots; ++k) {
+        c[k] = word_of(b, kPair ? 2 * k : 2 + 2 * k);
+        const uint32_t wk = word_of(b, kPair ? 2 * k + 1 : 3 + 2 * k);
         nd[k] = d + wk;
-        bool e = (uint32_t)k < ideg;
+        const uint32_t eidx = kPair ? (uint32_t)k + (half ? 3u : 0xFFFFFFFFu) : (uint32_t)k;
+        bool e = eidx < ideg;
         if ((kExt && a.light_split())) {
             const bool hv = wk >= a.q.delta;
             need_heavy = need_heavy || (e && hv && !hphase);
             e = e && hv == hphase;
         }
         if (e) on |= 1u << k;
+    }
+    if (kPair && (kExt && a.light_split())) {
+        // the odd lane's heavy inline edges also need the heavy-phase record
+        const uint32_t nh = need_heavy ? 1u : 0u;
+        const uint32_t other = dpp<0xF5>(nh);   // quad_perm [1,1,3,3]
+        need_heavy = lead & ((nh | other) != 0u);
     }
     // Block-carrying L1: load every relaxed child's whole block in this
     // round trip; the improved children take it into L1 with them, and the
@@ -446,17 +485,17 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     }
     uint32_t go = on;   // bit k: edge k issues its atomicMin
     if (kPre) {
-        uint32_t pre[kInlineEdges];
+        uint32_t pre[kSlots];
 #pragma unroll
-        for (int k = 0; k < (int)kInlineEdges; ++k) pre[k] = ((on >> k) & 1) ? a.dist[c[k]] : 0u;
+        for (int k = 0; k < kSlots; ++k) pre[k] = ((on >> k) & 1) ? a.dist[c[k]] : 0u;
 #pragma unroll
-        for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pre[k]));
+        for (int k = 0; k < kSlots; ++k) asm volatile("" : "+v"(pre[k]));
 #pragma unroll
-        for (int k = 0; k < (int)kInlineEdges; ++k)
+        for (int k = 0; k < kSlots; ++k)
             if (nd[k] >= pre[k]) go &= ~(1u << k);
     }
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) {
+    for (int k = 0; k < kSlots; ++k) {
         pv[k] = 0;
         if ((go >> k) & 1) pv[k] = fetch_min(a.dist + c[k], nd[k]);
     }
@@ -469,7 +508,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // atomicMin described above.
     const bool late_chk = (kHot && hot && a.stale_check()) || (valid && a.stale_check() == 3);
     uint32_t cur2 = kUnreached;
-    if (late_chk) cur2 = fetch_min(a.dist + v, d);
+    if (late_chk && half == 0) cur2 = fetch_min(a.dist + v, d);
     // Prefetch, in the same round trip as the atomics: touch every relaxed
     // child's hop block, so the batch that later pops the child finds the
     // line in this XCD's L2 (its dist line is already there from the
@@ -485,7 +524,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // batch's block loads miss the L2 more often).
     if (!kHot) {
 #pragma unroll
-        for (int k = 0; k < (int)kInlineEdges; ++k) {
+        for (int k = 0; k < kSlots; ++k) {
             const uint32_t t = (a.prefetch() && ((on >> k) & 1)) ? c[k] : 0u;
             sunk ^= pf[k];
             pf[k] = reinterpret_cast<const uint32_t *>(a.blocks + 4ull * t)[0];
@@ -495,18 +534,19 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // compiler sinks each comparison into its predicated block and waits
     // for the atomics one at a time.
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) asm volatile("" : "+v"(pv[k]));
+    for (int k = 0; k < kSlots; ++k) asm volatile("" : "+v"(pv[k]));
     asm volatile("" : "+v"(cur2));
-    uint64_t out[kInlineEdges];
-    uint32_t bits = 0;   // bit k: edge k lowered its destination's distance
+    if (kPair) cur2 = dpp<0xA0>(cur2);
+    uint64_t out[kSlots];
+    uint32_t bits = 0;   // bit k: slot k's edge lowered its destination's distance
 #pragma unroll
-    for (int k = 0; k < (int)kInlineEdges; ++k) {
+    for (int k = 0; k < kSlots; ++k) {
         if (((go >> k) & 1) && nd[k] < pv[k]) bits |= 1u << k;
         out[k] = make_rec(c[k], nd[k]);
     }
     {
         const bool late_stale = late_chk && cur2 < d;
-        st.v[kStatStale] += popc(ballot(late_stale));
+        st.v[kStatStale] += popc(ballot(late_stale && half == 0));
         if (late_stale) {
             bits = 0;
             heavy = false;
@@ -523,11 +563,11 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     bool ok = true;
     if (sums >> 16) {
         if (kSLF) {
-            ok = l1_push<kInlineEdges, kHot ? 1 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
-                                                     st, blk, cb);
+            ok = l1_push<kSlots, kHot ? 1 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
+                                               st, blk, cb);
         } else {
 #pragma unroll
-            for (int k = 0; k < (int)kInlineEdges; ++k)
+            for (int k = 0; k < kSlots; ++k)
                 if (ok) ok = l2_push(a.q, out[k], (bits >> k) & 1, st);
         }
         clk.lap(st, kClkFlush);
@@ -546,14 +586,16 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 // The worker loop of one wave for one query (reference work_block_kernel
 // loop, SS/SSSP/sssp_run.cu:507-584).  Returns when the query's exit flag is
 // seen, after an error, or at the deadline.  `leader` waves also run the
-// termination check while idle.
+// termination check while idle.  With lane pairs (kPair) a batch holds at
+// most 32 records, two lanes each: pops and L2 top-ups take at most 32.
 template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false, bool kPre = false,
-          uint32_t kCap = kL1Cap>
+          uint32_t kCap = kL1Cap, bool kPair = false>
 __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
                            uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
                            uint64_t deadline, HopBlock *blk = nullptr) {
     const uint32_t lane = lane_id();
     const L2Queue &q = a.q;
+    constexpr uint32_t kRecs = kPair ? kWave / 2 : kWave;   // records a batch can hold
     L1State l1{0u, 0u};
     // Records taken from L2 whose work is not finished.  Credits are returned
     // only when the wave runs out of work (the idle path below): a wave with
@@ -584,6 +626,9 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         if (kHot) {
             nin = l1_pop_hot(ring, blk, l1, a.pop_max(), rec, hot, hw);
             clk.lap(st, kClkL1Pop);
+        } else if (kPair) {
+            nin = l1_pop_pair<kCap>(ring, l1, min(a.pop_max(), kRecs), rec);
+            clk.lap(st, kClkL1Pop);
         } else if (kSLF) {
             nin = l1_pop<kCap>(ring, l1, a.pop_max(), rec);
             clk.lap(st, kClkL1Pop);
@@ -591,12 +636,13 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         // Top up from L2 below min_read.  A block-carrying L1 may hold back
         // cold records behind a hot run; it tops up only once it is empty, so
         // a hot batch stays free of global loads.
-        if (nin < a.min_read() && (!kHot || l1.size == 0)) {
+        if (nin < min(a.min_read(), kRecs) && (!kHot || l1.size == 0)) {
             const uint32_t got =
-                l2_pop<kExt>(q, cl_start, cl_mask, stage, kWave - nin, st, base_hint);
+                l2_pop<kExt>(q, cl_start, cl_mask, stage, kRecs - nin, st, base_hint);
             if (got) {
                 credit += got;
-                if (lane >= nin && lane < nin + got) rec = stage[lane - nin];
+                const uint32_t r = kPair ? lane >> 1 : lane;
+                if (r >= nin && r < nin + got) rec = stage[r - nin];
                 nin += got;
             }
             clk.lap(st, kClkL2Pop);
@@ -630,7 +676,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf, kHot, kExt, kPre, kCap>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
+        if (!process_batch<kSLF, kProf, kHot, kExt, kPre, kCap, kPair>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
                                               st, clk))
             break;
         if (kSLF && hungry && l1.size > a.donate_min()) {
@@ -667,7 +713,7 @@ __device__ __forceinline__ void flush_stats(uint32_t *ctl, WaveStats &st) {
 // kHot: block-carrying L1 (each wave's ring paired with 512 hop-block slots in
 // LDS, 32 KiB per wave; about 150 KiB per 4-wave workgroup, so one
 // workgroup per CU).
-template <int kWaves, bool kSLF, bool kProf, bool kHot, bool kExt = false>
+template <int kWaves, bool kSLF, bool kProf, bool kHot, bool kExt = false, bool kPair = false>
 __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs a) {
     __shared__ uint64_t s_ring[kWaves][kL1Alloc];
     __shared__ uint32_t s_map[kWaves][kWave];
@@ -686,7 +732,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_persistent_kernel(SolveArgs 
         for (int i = 0; i < 16; ++i) clk.lap(st, kClkLaps);
     }
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<kSLF, kProf, kHot, kExt>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline,
+    run_worker<kSLF, kProf, kHot, kExt, false, kL1Cap, kPair>(a, gwave == 0, s_ring[w], s_map[w], s_stage[w], st, clk, deadline,
                                   kHot ? reinterpret_cast<HopBlock *>(s_blk[w]) : nullptr);
     if (kProf) st.clk[kClkTotal] = clock64() - t_begin;
     flush_stats(a.q.ctl, st);
@@ -717,7 +763,8 @@ struct BatchArgs {
 // kStats = false leaves the statistics out: the counters then need no
 // registers (the per-wave counters otherwise spill into VGPR lanes and bound
 // the waves per SIMD at 4; without them it is 5).
-template <int kWaves, bool kProf, bool kExt = false, uint32_t kCap = kL1Cap, bool kStats = true>
+template <int kWaves, bool kProf, bool kExt = false, uint32_t kCap = kL1Cap, bool kStats = true,
+          bool kPair = false>
 __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
     __shared__ uint64_t s_ring[kWaves][kCap + 64];
     __shared__ uint32_t s_map[kWaves][kWave];
@@ -753,7 +800,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
         __threadfence();
         __syncthreads();
         a.dist = b.dist_all + (uint64_t)qi * a.n;
-        run_worker<true, kProf, false, kExt, true, kCap>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st,
+        run_worker<true, kProf, false, kExt, true, kCap, kPair>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st,
                                                    clk, deadline);
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -843,7 +890,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(ServeRequest *r
     st.clear();
     PhaseClock<false> clk;
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<true, false, false, kExt, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
+    run_worker<true, false, false, kExt, true, kL1Cap, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
                                          clk, deadline);
     __threadfence();
     __syncthreads();
@@ -1082,7 +1129,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_stream_kernel(StreamArgs s) 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         // 4. the worker loop
         if (s_go) {
-            run_worker<true, false, false, true, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w],
+            run_worker<true, false, false, true, true, kL1Cap, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w],
                                                      st, clk, deadline);
         } else if (threadIdx.x == 0) {
             cas32(&sy->status, kErrNone, kErrTimeout);
@@ -1301,33 +1348,43 @@ hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, h
     return hipGetLastError();
 }
 
-static const void *solver_fn(bool slf, bool hot) {
+static const void *solver_fn(bool slf, bool hot, bool pair) {
     if (slf && hot)
         return reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true, false, true>);
+    if (slf && pair)
+        return reinterpret_cast<const void *>(
+            &sssp_persistent_kernel<kWavesPerBlock, true, false, false, false, true>);
     return slf ? reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, true, false, false>)
                : reinterpret_cast<const void *>(&sssp_persistent_kernel<kWavesPerBlock, false, false, false>);
 }
 
-hipError_t solver_occupancy(bool slf, bool hot, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, solver_fn(slf, hot),
+hipError_t solver_occupancy(bool slf, bool hot, bool pair, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, solver_fn(slf, hot, pair),
                                                         kWavesPerBlock * kWave, 0);
 }
 
-template <int kW, bool kExt>
-static const void *batch_fn(int cap, bool stats) {
-    if (cap == 1024)
-        return stats ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, 1024, true>)
-                     : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, 1024, false>);
-    return stats ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, kL1Cap, true>)
-                 : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, kL1Cap, false>);
+template <int kW, bool kExt, uint32_t kCap, bool kPair>
+static const void *batch_fn_pair(bool stats) {
+    return stats ? reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, kCap, true, kPair>)
+                 : reinterpret_cast<const void *>(&sssp_batch_kernel<kW, false, kExt, kCap, false, kPair>);
 }
 
-hipError_t batch_occupancy(int waves, int l1_cap, bool ext, bool stats, int *blocks_per_cu) {
-    const void *fn = waves == 16 ? (ext ? batch_fn<16, true>(l1_cap, stats) : batch_fn<16, false>(l1_cap, stats))
-                   : waves == 8  ? (ext ? batch_fn<8, true>(l1_cap, stats) : batch_fn<8, false>(l1_cap, stats))
-                                 : (ext ? batch_fn<4, true>(l1_cap, stats) : batch_fn<4, false>(l1_cap, stats));
+template <int kW, bool kExt>
+static const void *batch_fn(int cap, bool stats, bool pair) {
+    if (cap == 1024)
+        return pair ? batch_fn_pair<kW, kExt, 1024, true>(stats) : batch_fn_pair<kW, kExt, 1024, false>(stats);
+    return pair ? batch_fn_pair<kW, kExt, kL1Cap, true>(stats) : batch_fn_pair<kW, kExt, kL1Cap, false>(stats);
+}
+
+hipError_t batch_occupancy(int waves, int l1_cap, bool ext, bool stats, bool pair, int *blocks_per_cu) {
+    const void *fn =
+        waves == 16 ? (ext ? batch_fn<16, true>(l1_cap, stats, pair) : batch_fn<16, false>(l1_cap, stats, pair))
+        : waves == 8 ? (ext ? batch_fn<8, true>(l1_cap, stats, pair) : batch_fn<8, false>(l1_cap, stats, pair))
+                     : (ext ? batch_fn<4, true>(l1_cap, stats, pair) : batch_fn<4, false>(l1_cap, stats, pair));
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kWave, 0);
 }
+
+static bool lane_pairs(const SolverLaunch &L) { return lane_pairs_for(L.pop_max); }
 
 // Kernel arguments from a host launch description.
 static SolveArgs make_args(const SolverLaunch &L) {
@@ -1365,25 +1422,30 @@ hipError_t launch_batch_reset(uint32_t *dist_all, uint32_t n, uint32_t nq, const
     return hipGetLastError();
 }
 
-template <int kW, bool kExt, uint32_t kCap>
+template <int kW, bool kExt, uint32_t kCap, bool kPair>
 static void launch_batch_instance(const dim3 &grid, const BatchArgs &b, bool stats, hipStream_t s) {
     if (stats)
-        hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap, true>), grid, dim3(kW * kWave), 0, s, b);
+        hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap, true, kPair>), grid, dim3(kW * kWave), 0,
+                           s, b);
     else
-        hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap, false>), grid, dim3(kW * kWave), 0, s, b);
+        hipLaunchKernelGGL((sssp_batch_kernel<kW, false, kExt, kCap, false, kPair>), grid, dim3(kW * kWave), 0,
+                           s, b);
 }
 
 template <int kW, bool kExt>
-static void launch_batch_cap(const dim3 &grid, const BatchArgs &b, int cap, bool stats, hipStream_t s) {
-    if (cap == 1024) launch_batch_instance<kW, kExt, 1024>(grid, b, stats, s);
-    else launch_batch_instance<kW, kExt, kL1Cap>(grid, b, stats, s);
+static void launch_batch_cap(const dim3 &grid, const BatchArgs &b, int cap, bool stats, bool pair,
+                             hipStream_t s) {
+    if (cap == 1024 && pair) launch_batch_instance<kW, kExt, 1024, true>(grid, b, stats, s);
+    else if (cap == 1024) launch_batch_instance<kW, kExt, 1024, false>(grid, b, stats, s);
+    else if (pair) launch_batch_instance<kW, kExt, kL1Cap, true>(grid, b, stats, s);
+    else launch_batch_instance<kW, kExt, kL1Cap, false>(grid, b, stats, s);
 }
 
 template <int kW>
 static void launch_batch_ext(const dim3 &grid, const BatchArgs &b, bool ext, int cap, bool stats,
-                             hipStream_t s) {
-    if (ext) launch_batch_cap<kW, true>(grid, b, cap, stats, s);
-    else launch_batch_cap<kW, false>(grid, b, cap, stats, s);
+                             bool pair, hipStream_t s) {
+    if (ext) launch_batch_cap<kW, true>(grid, b, cap, stats, pair, s);
+    else launch_batch_cap<kW, false>(grid, b, cap, stats, pair, s);
 }
 
 hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t s) {
@@ -1399,9 +1461,10 @@ hipError_t launch_batch(const SolverLaunch &L, const BatchLaunch &B, hipStream_t
     b.qtime = B.qtime;
     const dim3 grid(L.grid_blocks);
     const bool ext = L.light_split || L.defer_heavy || L.strict_window;
-    if (B.waves == 16) launch_batch_ext<16>(grid, b, ext, B.l1_cap, B.stats, s);
-    else if (B.waves == 8) launch_batch_ext<8>(grid, b, ext, B.l1_cap, B.stats, s);
-    else launch_batch_ext<4>(grid, b, ext, B.l1_cap, B.stats, s);
+    const bool pair = lane_pairs(L);
+    if (B.waves == 16) launch_batch_ext<16>(grid, b, ext, B.l1_cap, B.stats, pair, s);
+    else if (B.waves == 8) launch_batch_ext<8>(grid, b, ext, B.l1_cap, B.stats, pair, s);
+    else launch_batch_ext<4>(grid, b, ext, B.l1_cap, B.stats, pair, s);
     return hipGetLastError();
 }
 
@@ -1516,7 +1579,20 @@ hipError_t launch_solver(const SolverLaunch &L, hipStream_t s) {
     const dim3 grid(L.grid_blocks), block(kWavesPerBlock * kWave);
     const bool hot = L.slf && L.hot;
     const bool ext = L.light_split || L.defer_heavy || L.strict_window;
-    if (L.slf && hot && !L.profile)
+    const bool pair = L.slf && !hot && lane_pairs(L);
+    if (pair && !L.profile && ext)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, false, true, true>), grid, block,
+                           0, s, a);
+    else if (pair && !L.profile)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, false, false, true>), grid, block,
+                           0, s, a);
+    else if (pair && ext)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, false, true, true>), grid, block, 0,
+                           s, a);
+    else if (pair)
+        hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, false, false, true>), grid, block,
+                           0, s, a);
+    else if (L.slf && hot && !L.profile)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, false, true>), grid, block, 0, s, a);
     else if (L.slf && hot)
         hipLaunchKernelGGL((sssp_persistent_kernel<kWavesPerBlock, true, true, true>), grid, block, 0, s, a);
