@@ -81,7 +81,10 @@ typedef struct mlmq_params {
     double   timeout_s;       /* device watchdog; <= 0 means 60 s */
     int32_t  auto_retry;      /* on MLMQ_E_QUEUE_OVERFLOW grow capacity and rerun (default 1) */
     int32_t  profile;         /* 1: per-phase clock build of the kernel (mlmq_timing.phase_cycles) */
-    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 32 */
+    int32_t  l1_pop;          /* records a wave takes from its L1 per batch, 1..64, default 32.
+                                 Up to 32 (record-only L1): two lanes per record, each relaxing
+                                 half of its hop block, and L2 top-ups also stop at 32 records
+                                 (min_read above 32 acts as 32); 33..64: one lane per record */
     int32_t  l1_spill;        /* L1 holding more records than this after a push spills to L2;
                                  1..l1_ring - 448 (64 with the default ring), default 64 */
     int32_t  l1_donate;       /* while some waves are idle, a wave whose L1 holds more than this

@@ -92,6 +92,21 @@ def test_light_split_batch(solver):
                 buckets=32)
 
 
+@pytest.mark.parametrize("pop", [64, 32])
+@pytest.mark.parametrize("waves", [4, 16])
+def test_batch_lane_pairs_and_single_lanes(solver, pop, waves):
+    """Pops of up to 32 records run two lanes per record, 64 one lane per
+    record; both exact, with and without the light/heavy split, with top-up
+    thresholds above 32."""
+    say(f"batch pop {pop} waves {waves}")
+    row, col, w = random_csr(6000, 40000, 31, wmax=1000, zero_frac=0.1)
+    solver.load_csr(row, col, w)
+    srcs = [0, 17, 5999, 3000, 17]
+    check_batch(solver, row, col, w, srcs, batch_waves=waves, l1_pop=pop, delta=300)
+    check_batch(solver, row, col, w, srcs, batch_waves=waves, l1_pop=pop, delta=100,
+                light_split=1, buckets=32, min_read=64, l2_batch=64)
+
+
 def test_strict_window_batch(solver):
     say("strict window batch")
     row, col, w = hub_csr(20000, [40000, 5000, 2100], 40000, seed=9)

@@ -201,6 +201,34 @@ def test_rmat_small(solver):
         solve_check(solver, row, col, w, src, **cfg)
 
 
+@pytest.mark.parametrize("pop", [32, 16, 64])
+def test_lane_pairs_and_single_lanes(solver, pop):
+    """Pops of up to 32 records run two lanes per record (each relaxes half
+    of the hop block, the even lane owns CSR edges, stale checks and
+    heavy-phase records); 64 runs one lane per record.  Both are exact on
+    graphs with degrees across the 7-edge block boundary, light/heavy
+    splits (inline heavy edges on either lane of a pair), hubs, zero
+    weights, and with top-up thresholds above 32."""
+    say(f"lane pairs / single lanes, pop {pop}")
+    cases = [
+        (random_csr(3000, 20000, 21, wmax=1000, zero_frac=0.1), 0),
+        (grid_csr(60, 70, 22), 5),
+        (hub_csr(6000, [5000, 3000, 900], 12000, seed=23, wmax=1000), 1),
+    ]
+    g = HostGraph.rmat(15, 12, seed=24)
+    rmat = tuple(x.copy() for x in g.arrays())
+    cases.append((rmat, g.max_degree_vertex()))
+    g.close()
+    for (row, col, w), src in cases:
+        solver.load_csr(row, col, w)
+        for cfg in (dict(), dict(light_split=1, delta=100, buckets=32, bucket_window=2),
+                    dict(light_split=1, delta=300, stale_check=3),
+                    dict(stale_check=1, min_read=64, l2_batch=64, delta=500),
+                    dict(queue="L1V_L2DQ", light_split=1, delta=200),
+                    dict(hub_split=1, delta=1000, prefetch=0)):
+            solve_check(solver, row, col, w, src, load=False, l1_pop=pop, **cfg)
+
+
 @pytest.mark.parametrize("split", [False, True])
 def test_bfs_renumbered_graph(solver, split):
     say(f"BFS-renumbered road-like graph, layer split {split}")
