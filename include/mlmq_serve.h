@@ -100,8 +100,7 @@ typedef struct mlmq_serve_graph_params {
     int32_t  bucket_window;   /* 1..buckets; 0 = 4 */
     int32_t  l2_batch;        /* 1..64; 0 = 8 */
     int32_t  min_read;        /* 1..64; 0 or -1 = 8 */
-    int32_t  l1_pop;          /* 1..64; 0 = 32. Served requests always run two lanes per
-                                 record, so a pop takes at most 32 records */
+    int32_t  l1_pop;          /* 1..64; 0 = 32 (up to 32: two lanes per record, as in mlmq_solve) */
     int32_t  l1_spill;        /* 1..64; 0 = 64 */
     int32_t  workgroups;      /* workgroups per request, 1..64; 0 = auto from the edge count */
     int32_t  waves;           /* waves per workgroup: 1, 4, 8 or 16; 0 = 8 */

@@ -152,8 +152,8 @@ hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const u
                                uint32_t n, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);
 // Lane pairs (two lanes per record, four edge slots each) whenever a batch
-// pops at most 32 records; the serving kernels always use them.
-inline bool lane_pairs_for(uint32_t pop_max) { return pop_max <= 32u; }
+// pops at most 32 records (a serving launch decides per request).
+__host__ __device__ inline bool lane_pairs_for(uint32_t pop_max) { return pop_max <= 32u; }
 hipError_t solver_occupancy(bool slf, bool hot, bool pair, int *blocks_per_cu);
 hipError_t launch_solver(const SolverLaunch &L, hipStream_t s);
 

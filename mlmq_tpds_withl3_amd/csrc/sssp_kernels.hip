@@ -890,8 +890,13 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_serve_kernel(ServeRequest *r
     st.clear();
     PhaseClock<false> clk;
     const uint64_t deadline = wall_clock64() + a.timeout_ticks;
-    run_worker<true, false, false, kExt, true, kL1Cap, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w], st,
-                                         clk, deadline);
+    // lane pairs per request, from its own pop size (uniform in the workgroup)
+    if (lane_pairs_for(a.pop_max()))
+        run_worker<true, false, false, kExt, true, kL1Cap, true>(a, part == 0 && w == 0, s_ring[w], s_map[w],
+                                                                 s_stage[w], st, clk, deadline);
+    else
+        run_worker<true, false, false, kExt, true, kL1Cap, false>(a, part == 0 && w == 0, s_ring[w], s_map[w],
+                                                                  s_stage[w], st, clk, deadline);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1129,8 +1134,12 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_stream_kernel(StreamArgs s) 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         // 4. the worker loop
         if (s_go) {
-            run_worker<true, false, false, true, true, kL1Cap, true>(a, part == 0 && w == 0, s_ring[w], s_map[w], s_stage[w],
-                                                     st, clk, deadline);
+            if (lane_pairs_for(a.pop_max()))
+                run_worker<true, false, false, true, true, kL1Cap, true>(a, part == 0 && w == 0, s_ring[w], s_map[w],
+                                                                         s_stage[w], st, clk, deadline);
+            else
+                run_worker<true, false, false, true, true, kL1Cap, false>(a, part == 0 && w == 0, s_ring[w], s_map[w],
+                                                                          s_stage[w], st, clk, deadline);
         } else if (threadIdx.x == 0) {
             cas32(&sy->status, kErrNone, kErrTimeout);
             raise_error(a.q, kErrTimeout);
