@@ -95,7 +95,8 @@ typedef struct mlmq_params {
                                  (default 1) */
     int32_t  hub_split;       /* 1: a popped hub (> 2048 out-edges) keeps its first 1024-edge
                                  chunk and sends the rest to L2 as chunk records that other
-                                 waves relax; 0 (default): one wave relaxes the whole list */
+                                 waves relax (with light_split: the heavy-phase part only);
+                                 0 (default): one wave relaxes the whole list */
     int32_t  stale_check;     /* filter of popped records whose distance has since dropped:
                                  2 (default) coherent read of dist[v] with the hop block, 1 load
                                  through the L2 (may miss a drop made on another XCD), 3 late:

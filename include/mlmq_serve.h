@@ -108,7 +108,7 @@ typedef struct mlmq_serve_graph_params {
                                  Doubled automatically when a request overflows (the request
                                  is then run again). */
     int32_t  hub_split;       /* 1: split hub edge lists into chunk records (mlmq_params.hub_split) */
-    int32_t  light_split;     /* 1: light/heavy edge split (mlmq_params.light_split); excludes hub_split */
+    int32_t  light_split;     /* 1: light/heavy edge split (mlmq_params.light_split) */
     int32_t  prefetch;        /* 1 = touch relaxed children's hop blocks (mlmq_params.prefetch),
                                  -1 = don't; 0 = default (1) */
     int32_t  stale_check;     /* mlmq_params.stale_check: 2 coherent, 1 through the L2, 3 late,

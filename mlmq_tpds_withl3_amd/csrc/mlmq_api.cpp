@@ -411,8 +411,7 @@ int check_params(const mlmq_params &p, bool batch, std::string *why) {
     if (p.idle_backoff < 0 || p.idle_backoff > 64) return bad(MLMQ_E_INVALID, "idle_backoff must be in [0, 64]");
     if (p.csr_stream != 0 && p.csr_stream != 1) return bad(MLMQ_E_INVALID, "csr_stream must be 0 or 1");
     if (p.batch_stats != 0 && p.batch_stats != 1) return bad(MLMQ_E_INVALID, "batch_stats must be 0 or 1");
-    if (p.light_split && (p.hub_split || p.defer_heavy))
-        return bad(MLMQ_E_INVALID, "light_split excludes hub_split and defer_heavy");
+    if (p.light_split && p.defer_heavy) return bad(MLMQ_E_INVALID, "light_split excludes defer_heavy");
     if (p.l1_blocks && (p.light_split || p.defer_heavy || p.strict_window))
         return bad(MLMQ_E_INVALID, "l1_blocks excludes light_split, defer_heavy and strict_window");
     if (p.work_blocks < 0 || p.blocks_per_cu < 0 || p.bucket_capacity < 0)

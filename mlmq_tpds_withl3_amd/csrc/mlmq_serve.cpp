@@ -185,7 +185,6 @@ int resolve_params_why(const mlmq_serve_graph_params *in, int32_t n, int64_t m,
     if (p.bucket_capacity < 0) return fail(MLMQ_E_INVALID, "negative bucket_capacity");
     if (p.hub_split != 0 && p.hub_split != 1) return fail(MLMQ_E_INVALID, "hub_split must be 0 or 1");
     if (p.light_split != 0 && p.light_split != 1) return fail(MLMQ_E_INVALID, "light_split must be 0 or 1");
-    if (p.light_split && p.hub_split) return fail(MLMQ_E_INVALID, "light_split excludes hub_split");
     if (p.prefetch < -1 || p.prefetch > 1) return fail(MLMQ_E_INVALID, "prefetch must be -1, 0 or 1");
     if (p.stale_check < -1 || p.stale_check > 3)
         return fail(MLMQ_E_INVALID, "stale_check must be -1, 0, 1, 2 or 3");

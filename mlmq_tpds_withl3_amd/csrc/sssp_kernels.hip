@@ -419,20 +419,14 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     uint32_t hstart = b.start();
     uint32_t nextra = 0;   // chunks of a hub popped here that go to L2
     const uint32_t hub_word = b.q[0].z, csize = b.q[0].w;
-    if (chunk) {
-        const uint32_t lo = ci * csize;
-        deg = lo < deg ? min(csize, deg - lo) : 0u;
-        hstart += lo;
-    } else if (a.split() && deg > kInlineEdges && hub_word != 0) {
-        nextra = (deg - 1) / csize;
-        deg = min(deg, csize);
-    }
     // chunk lanes always take the CSR path (their edges are not inline)
     const bool csr_path = chunk || bdeg > kInlineEdges;
-    if (half) nextra = 0;   // (the odd lane's header words are edges)
     bool need_heavy = false;   // a light-phase record whose vertex has heavy edges
-    if ((kExt && a.light_split()) && csr_path && !chunk) {
-        // adjacency sorted by weight: the first b.q[1].x edges are light
+    // The light/heavy split first: the record's phase selects its part of the
+    // adjacency (sorted by weight: the first b.q[1].x edges are light).  A
+    // hub's part is then chunked like a whole list; with the split only the
+    // heavy part is (chunk records carry kHeavyPhase).
+    if ((kExt && a.light_split()) && csr_path) {
         const uint32_t nl = min(b.q[1].x, deg);
         if (hphase) {
             hstart += nl;
@@ -442,11 +436,22 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             deg = nl;
         }
     }
+    if (chunk) {
+        const uint32_t lo = ci * csize;
+        deg = lo < deg ? min(csize, deg - lo) : 0u;
+        hstart += lo;
+    } else if (a.split() && deg > kInlineEdges && hub_word != 0 &&
+               (!(kExt && a.light_split()) || hphase)) {
+        nextra = (deg - 1) / csize;
+        deg = min(deg, csize);
+    }
+    if (half) nextra = 0;   // (the odd lane's header words are edges)
     bool heavy = csr_path && deg > 0 && half == 0;
     const uint32_t ideg = csr_path ? 0u : deg;
     clk.lap(st, kClkRow);
     if (ballot(nextra > 0)) {
-        if (!push_chunks(a, hub_word - 1, d, nextra, map, st)) return false;
+        if (!push_chunks(a, hub_word - 1, d | (hphase ? kHeavyPhase : 0u), nextra, map, st))
+            return false;
     }
 
     // Slot k holds edge k; with lane pairs slot k holds the lane's words 2k

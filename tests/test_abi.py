@@ -150,7 +150,7 @@ def test_params_check_ranges():
     assert check(buckets=65) == -1 and check(buckets=4, bucket_window=5) == -1
     assert check(l2_batch=0) == -1 and check(l1_pop=65) == -1
     assert check(stale_check=4) == -1 and check(work_blocks=-1) == -1
-    assert check(light_split=1, hub_split=1) == -1
+    assert check(light_split=1, defer_heavy=1) == -1 and check(light_split=1, hub_split=1) == 0
     assert check(queue_mode=5) == -9
     assert check(queue_mode=0) == 0 and check(queue_mode=0, batch=1) == -9
     assert check(batch=1, batch_waves=3) == -1 and check(batch=1, batch_waves=16) == 0

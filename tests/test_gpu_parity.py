@@ -131,6 +131,13 @@ HUBS = [70000, 9000, 4097, 2049, 2048, 1500]   # degrees 70002 .. 1504: five hub
     dict(light_split=1, delta=500, queue="L1V_L2DQ", hub_split=0),
     dict(stale_check=3),                            # late check skips stale chunks' edges
     dict(stale_check=3, light_split=1, delta=100, hub_split=0),
+    # light/heavy split with hub splitting: a hub's heavy part goes out in chunks
+    dict(light_split=1, delta=100),
+    dict(light_split=1, delta=20, buckets=64, bucket_window=4),
+    dict(light_split=1, delta=500, queue="L1V_L2DQ"),
+    dict(light_split=1, stale_check=3, delta=100),
+    dict(light_split=1, stale_check=1, delta=50, l1_pop=64, min_read=64, l2_batch=64, buckets=64,
+         bucket_window=2),
 ])
 def test_hub_split(solver, cfg):
     say(f"hubs {cfg}")
