@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--extended-sweep", type=int, default=1,
                     help="single mode: also time deltas 2M and 20M (outside the configured sweep)")
     ap.add_argument("--batch-waves", type=int, default=8)
+    ap.add_argument("--l1-ring", type=int, default=0,
+                    help="batch mode: L1 ring records per wave, 512 or 1024 (0: 512)")
     ap.add_argument("--source-seed", type=int, default=12345)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain copy kernel of this many bytes (counter calibration)")
@@ -454,6 +456,8 @@ def run_batch(args):
                 l1_pop=args.l1_pop, min_read=args.min_read, l2_batch=args.l2_batch,
                 buckets=args.buckets, bucket_window=args.window, prefetch=args.prefetch,
                 stale_check=args.stale_check, batch_stats=0)
+    if args.l1_ring:
+        base["l1_ring"] = args.l1_ring
     solver = Solver(0)
     solver.load_csr(row, col, w)
     tuned = None
