@@ -236,6 +236,21 @@ def test_lane_pairs_and_single_lanes(solver, pop):
             solve_check(solver, row, col, w, src, load=False, l1_pop=pop, **cfg)
 
 
+@pytest.mark.parametrize("cfg", [dict(), dict(l1_pop=64), dict(light_split=1, delta=100),
+                                 dict(light_split=1, delta=100, l1_pop=64),
+                                 dict(queue="L1N_L2DQ"), dict(l1_blocks=1)])
+def test_profiling_build(solver, cfg):
+    """profile=1 runs the per-phase clock instance of the kernel: the same
+    distances, and the phase clocks are filled."""
+    say(f"profiling build {cfg}")
+    row, col, w = hub_csr(5000, [3000, 900], 15000, seed=41, wmax=1000)
+    solver.load_csr(row, col, w)
+    t, _, _ = solve_check(solver, row, col, w, 1, load=False, profile=1, **cfg)
+    ph = t["phase_cycles"]
+    assert ph["total"] > 0 and ph["l1_pop" if cfg.get("queue") != "L1N_L2DQ" else "l2_pop"] >= 0
+    assert sum(v for k, v in ph.items() if k != "total") > 0
+
+
 @pytest.mark.parametrize("split", [False, True])
 def test_bfs_renumbered_graph(solver, split):
     say(f"BFS-renumbered road-like graph, layer split {split}")
