@@ -151,7 +151,9 @@ typedef struct mlmq_timing {
     uint64_t l2_pushes;       /* records reserved in L2 (incl. the seed) */
     uint64_t l2_pops;         /* records taken from L2 */
     uint64_t relaxations;     /* edges examined */
-    uint64_t atomics;         /* atomicMin issued */
+    uint64_t atomics;         /* atomicMin issued: edges examined less those skipped by the
+                                 heavy loop's cached pre-check or as a record's back edge
+                                 (the edge to the vertex whose expansion created it) */
     uint64_t improvements;    /* atomicMin that lowered a distance (= records pushed) */
     uint64_t stale;           /* popped records skipped as stale */
     uint64_t batches;         /* processed record batches */

@@ -19,6 +19,15 @@
 // take them.  A chunk record has the hub's distance and, in the vertex word,
 // kChunkFlag | hub id << kChunkBits | chunk index; the hub table maps hub ids
 // back to vertices.  Vertex ids are below 2^31, so the flag bit is free.
+//
+// Back edges: when every vertex id and weight is below 2^kBackShift (a
+// "packed" graph), the destination word of inline edge k of v, to u, also
+// holds in bits kBackShift..30 the slot + 1 of u's first inline edge back to
+// v (0: none).  A record created by relaxing that edge keeps the word as its
+// vertex word, and the expansion of the record skips that slot: an edge back to the vertex p whose expansion (at distance d_p)
+// created the record offers d_p + w(p, u) + w(u, p) >= d_p >= dist[p], which
+// never lowers dist[p].  On the road-like inputs this is one of about three
+// inline edges of every expanded vertex.
 #pragma once
 
 #include "device_common.h"
@@ -31,6 +40,7 @@ constexpr uint32_t kChunkEdges = 1024;      // default chunk of a hub's edge lis
 constexpr uint32_t kChunkFlag = 0x80000000u;
 constexpr uint32_t kChunkBits = 16;         // chunk index bits (chunks per hub < 2^16)
 constexpr uint32_t kMaxHubs = 1u << 15;     // hub id bits: 15
+constexpr uint32_t kBackShift = 28;         // back-edge slot bits: 28..30
 
 __device__ __host__ inline uint32_t chunk_vertex_word(uint32_t hub, uint32_t ci) {
     return kChunkFlag | (hub << kChunkBits) | ci;

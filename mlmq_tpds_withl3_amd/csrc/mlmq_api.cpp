@@ -36,6 +36,7 @@ struct mlmq_ctx {
     uint2 *hubs = nullptr;     // hub table (vertex, chunk size), owned
     uint32_t nhubs = 0;
     bool sorted = false;       // every adjacency sorted by weight (owned uploads)
+    bool packed = false;       // hop blocks carry back-edge slots (mlmq::kBackShift)
     uint32_t light_delta = 0;  // delta the blocks' light-edge counts were computed for
     bool owns_graph = false;
     mlmq_graph_info info{};
@@ -229,7 +230,13 @@ int validate_csr(mlmq_ctx *c, int32_t n, int64_t m, const int64_t *row, const in
 // below 2^kChunkBits.
 int build_blocks(mlmq_ctx *c) {
     HIP_TRY(c, hipMalloc(&c->blocks, 64 * (size_t)c->n));
-    HIP_TRY(c, mlmq::launch_build_blocks(c->row, c->col, c->wgt, (uint32_t)c->n, c->blocks, c->stream));
+    // Back-edge slots fit when vertex ids and weights stay below 2^kBackShift
+    // (the weight statistic is a float: rounding can only refuse a graph
+    // that would fit).
+    const double lim = (double)(1u << mlmq::kBackShift);
+    c->packed = (double)c->n < lim && (double)c->info.max_weight < lim;
+    HIP_TRY(c, mlmq::launch_build_blocks(c->row, c->col, c->wgt, (uint32_t)c->n, c->packed, c->blocks,
+                                         c->stream));
     std::vector<uint32_t> row((size_t)c->n + 1);
     HIP_TRY(c, hipMemcpyAsync(row.data(), c->row, sizeof(uint32_t) * row.size(), hipMemcpyDeviceToHost,
                               c->stream));
@@ -435,6 +442,7 @@ int mlmq::ctx_graph_view(const mlmq_ctx *c, DeviceGraphView *v) {
     v->wgt = c->wgt;
     v->blocks = c->blocks;
     v->hubs = c->hubs;
+    v->packed = c->packed;
     v->n = c->n;
     v->m = c->m;
     v->device = c->device;
@@ -513,6 +521,7 @@ mlmq::SolverLaunch query_launch(const mlmq_ctx *c, const QueryPlan &q, uint32_t 
     L.wgt = c->wgt;
     L.blocks = c->blocks;
     L.hubs = c->hubs;
+    L.packed = c->packed;
     L.split = p.hub_split != 0;
     L.stale_check = (uint32_t)p.stale_check;
     L.prefetch = p.prefetch ? 1u : 0u;
@@ -1002,6 +1011,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
         L.wgt = c->wgt;
         L.blocks = c->blocks;
         L.hubs = c->hubs;
+        L.packed = c->packed;
         L.split = p.hub_split != 0;
         L.stale_check = (uint32_t)p.stale_check;
         L.prefetch = p.prefetch ? 1u : 0u;
@@ -1055,7 +1065,7 @@ int mlmq_solve_batch(mlmq_ctx *c, const int32_t *sources, int32_t nq, const mlmq
             (uint64_t)(uintptr_t)c->hubs, (uint64_t)L.split, (uint64_t)L.stale_check,
             (uint64_t)L.prefetch, (uint64_t)L.max_sleep, (uint64_t)L.defer_heavy,
             (uint64_t)L.strict_window, (uint64_t)L.light_split, (uint64_t)L.csr_stream,
-            (uint64_t)l1_cap, (uint64_t)B.stats};
+            (uint64_t)l1_cap, (uint64_t)B.stats, (uint64_t)c->n, (uint64_t)c->packed};
         const bool use_graph = p.use_graph != 0;
         if (use_graph && (!c->b_graph || c->b_graph_key != key)) {
             if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);

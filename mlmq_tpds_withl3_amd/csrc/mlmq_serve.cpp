@@ -271,6 +271,7 @@ mlmq::SolverLaunch graph_launch(const GraphEntry &g, uint64_t timeout_ticks) {
     L.wgt = g.v.wgt;
     L.blocks = g.v.blocks;
     L.hubs = g.v.hubs;
+    L.packed = g.v.packed;
     L.split = g.p.hub_split != 0;
     L.stale_check = g.p.stale_check == 0 ? 2u : g.p.stale_check < 0 ? 0u : (uint32_t)g.p.stale_check;
     L.prefetch = g.p.prefetch >= 0 ? 1u : 0u;

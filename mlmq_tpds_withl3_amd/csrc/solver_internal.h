@@ -17,6 +17,7 @@ struct DeviceGraphView {
     const uint32_t *wgt;
     const uint4 *blocks;
     const uint2 *hubs;
+    bool packed;           // hop blocks carry back-edge slots
     int32_t n;
     int64_t m;
     int device;
@@ -45,6 +46,7 @@ struct SolverLaunch {
     const uint32_t *wgt;
     const uint4 *blocks;
     const uint2 *hubs;     // hub table (vertex, chunk size); null: no hubs
+    bool packed;           // hop blocks carry back-edge slots (graph property)
     uint32_t *dist;
     uint64_t *slots;
     uint32_t *ctl;
@@ -149,7 +151,7 @@ hipError_t launch_patch_hubs(const uint2 *hubs, uint32_t nh, uint4 *blocks, hipS
 hipError_t launch_light_count(const uint32_t *row, const uint32_t *wgt, uint32_t n, uint32_t delta,
                               uint4 *blocks, hipStream_t s);
 hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
-                               uint32_t n, uint4 *blocks, hipStream_t s);
+                               uint32_t n, bool packed, uint4 *blocks, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, uint64_t count, hipStream_t s);
 // Lane pairs (two lanes per record, four edge slots each) whenever a batch
 // pops at most 32 records (a serving launch decides per request).

@@ -38,7 +38,7 @@ struct SolveArgs {
     // persistent loop: lims = min_read | pop_max << 8 | spill << 16 |
     // max_sleep << 24 (each <= 64); flags = fifo | split << 1 |
     // stale_check << 2 | prefetch << 4 | defer_heavy << 5 | light_split << 6 |
-    // donate_min << 8 | csr_stream << 16.
+    // donate_min << 8 | csr_stream << 16 | packed << 17.
     uint32_t lims;
     uint32_t flags;
     __device__ __forceinline__ uint32_t min_read() const { return lims & 0xFFu; }
@@ -53,6 +53,10 @@ struct SolveArgs {
     __device__ __forceinline__ bool light_split() const { return (flags >> 6) & 1u; }
     __device__ __forceinline__ uint32_t donate_min() const { return (flags >> 8) & 0xFFu; }
     __device__ __forceinline__ bool csr_stream() const { return (flags >> 16) & 1u; }
+    // vertex-id and weight mask of a packed graph's records and hop blocks
+    __device__ __forceinline__ uint32_t id_mask() const {
+        return (flags >> 17) & 1u ? (1u << kBackShift) - 1u : 0xFFFFFFFFu;
+    }
     uint64_t timeout_ticks;
 };
 
@@ -342,6 +346,12 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // A chunk record names (hub id, chunk index); one extra round trip to the
     // hub table, taken only by batches that hold chunks.
     const bool chunk = valid && (v & kChunkFlag) != 0;
+    // Back edge (hop_block.h): on a packed graph the bits above the vertex id
+    // name the slot + 1 of the edge back to the vertex whose expansion
+    // created this record; that edge is not relaxed.
+    const uint32_t idm = a.id_mask();
+    const uint32_t vword = v;   // with the back-edge bits, for the heavy-phase record
+    const uint32_t back = chunk ? 0u : (v & ~idm) >> kBackShift;
     uint32_t ci = 0;
     if (ballot(chunk)) {
         if (chunk) {
@@ -349,6 +359,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             v = a.hubs[(v & ~kChunkFlag) >> kChunkBits].x;
         }
     }
+    if (!chunk) v &= idm;
     // Stale filter (reference :174, process on <=).  Distances only fall, so
     // a value read through this XCD's L2 (a.stale_check() == 1) is never below
     // the coherent one: the filter then lets some stale records through but
@@ -457,11 +468,14 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // Slot k holds edge k; with lane pairs slot k holds the lane's words 2k
     // and 2k + 1: edge k - 1 on the even lane (slot 0 is the header), edge
     // k + 3 on the odd lane.
-    uint32_t c[kSlots], nd[kSlots], pv[kSlots];
+    // cw[k]: the destination word as stored (on a packed graph with the
+    // child's back-edge bits, which its record keeps); c[k]: the vertex.
+    uint32_t cw[kSlots], c[kSlots], nd[kSlots], pv[kSlots];
     uint32_t on = 0;   // bit k: slot k's edge is relaxed by this record
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) {
-        c[k] = word_of(b, kPair ? 2 * k : 2 + 2 * k);
+        cw[k] = word_of(b, kPair ? 2 * k : 2 + 2 * k);
+        c[k] = cw[k] & idm;
         const uint32_t wk = word_of(b, kPair ? 2 * k + 1 : 3 + 2 * k);
         nd[k] = d + wk;
         const uint32_t eidx = kPair ? (uint32_t)k + (half ? 3u : 0xFFFFFFFFu) : (uint32_t)k;
@@ -473,6 +487,12 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
         }
         if (e) on |= 1u << k;
     }
+    // The back edge's slot bit (edge back - 1: slot back - 1, or with lane
+    // pairs slot back on the even lane for edges 0-2 and slot back - 4 on
+    // the odd lane for edges 3-6; back = 0 gives no bit).
+    const uint32_t seen = on;   // examined: relaxed or skipped as the back edge
+    on &= ~(kPair ? ((1u << back) >> (half ? 4u : 0u)) & (half ? 0xFu : 0xEu)
+                  : (1u << back) >> 1);
     if (kPair && (kExt && a.light_split())) {
         // the odd lane's heavy inline edges also need the heavy-phase record
         const uint32_t nh = need_heavy ? 1u : 0u;
@@ -547,7 +567,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) {
         if (((go >> k) & 1) && nd[k] < pv[k]) bits |= 1u << k;
-        out[k] = make_rec(c[k], nd[k]);
+        out[k] = make_rec(cw[k], nd[k]);
     }
     {
         const bool late_stale = late_chk && cur2 < d;
@@ -558,15 +578,20 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
             need_heavy = false;
         }
     }
-    // relaxations and improvements of the batch in one wave reduction
-    const uint32_t sums = wave_sum((uint32_t)__builtin_popcount(on) |
-                                   ((uint32_t)__builtin_popcount(bits) << 16));
-    st.v[kStatRelax] += sums & 0xFFFFu;
-    st.v[kStatImprove] += sums >> 16;
+    // Edges examined (relaxed, or skipped as back edges), skipped back edges
+    // and improvements of the batch in one wave reduction: 10-bit fields, at
+    // most 64 x 7 per batch.  (Skipped edges count with the atomics the heavy
+    // loop's pre-check skipped, see flush_stats.)
+    const uint32_t sums = wave_sum((uint32_t)__builtin_popcount(seen) |
+                                   ((uint32_t)__builtin_popcount(seen & ~on) << 10) |
+                                   ((uint32_t)__builtin_popcount(bits) << 20));
+    st.v[kStatRelax] += sums & 0x3FFu;
+    st.v[kStatAtomic] += (sums >> 10) & 0x3FFu;
+    st.v[kStatImprove] += sums >> 20;
     clk.lap(st, kClkAtomic);
 
     bool ok = true;
-    if (sums >> 16) {
+    if (sums >> 20) {
         if (kSLF) {
             ok = l1_push<kSlots, kHot ? 1 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
                                                st, blk, cb);
@@ -584,7 +609,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     // The heavy edges wait for a heavy-phase record one bucket later: by then
     // the light phase of this bucket has settled the vertex's distance, so
     // the heavy edges are relaxed once instead of once per improvement.
-    if (ok && ballot(need_heavy)) ok = l2_push(a.q, make_rec(v, d | kHeavyPhase), need_heavy, st);
+    if (ok && ballot(need_heavy)) ok = l2_push(a.q, make_rec(vword, d | kHeavyPhase), need_heavy, st);
     return ok;
 }
 
@@ -1219,9 +1244,12 @@ __global__ void batch_reset_kernel(uint32_t *dist_all, uint32_t n, uint32_t nq,
     if (tid < kCtlWords) gctl[tid] = 0u;
 }
 
-// Hop blocks from the CSR (layout in hop_block.h).
+// Hop blocks from the CSR (layout in hop_block.h).  With `packed`, the
+// destination word of inline edge k (v -> u) also carries, above kBackShift,
+// the slot + 1 of the first edge u -> v in u's block when u is inline too
+// (0: u has no such inline edge).
 __global__ void build_blocks_kernel(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
-                                    uint32_t n, uint4 *blocks) {
+                                    uint32_t n, uint32_t packed, uint4 *blocks) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
         const uint32_t s = row[v], deg = row[v + 1] - s;
@@ -1232,7 +1260,15 @@ __global__ void build_blocks_kernel(const uint32_t *row, const uint32_t *col, co
         wd[1] = s;
         if (deg <= kInlineEdges) {
             for (uint32_t k = 0; k < deg; ++k) {
-                wd[2 + 2 * k] = col[s + k];
+                const uint32_t u = col[s + k];
+                uint32_t back = 0;
+                if (packed) {
+                    const uint32_t su = row[u], du = row[u + 1] - su;
+                    if (du <= kInlineEdges)
+                        for (uint32_t j = du; j-- > 0;)
+                            if (col[su + j] == v) back = j + 1;
+                }
+                wd[2 + 2 * k] = u | back << kBackShift;
                 wd[3 + 2 * k] = wgt[s + k];
             }
         }
@@ -1337,9 +1373,9 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t count, uint32_t value, hipStrea
 }
 
 hipError_t launch_build_blocks(const uint32_t *row, const uint32_t *col, const uint32_t *wgt,
-                               uint32_t n, uint4 *blocks, hipStream_t s) {
+                               uint32_t n, bool packed, uint4 *blocks, hipStream_t s) {
     hipLaunchKernelGGL(build_blocks_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, row, col, wgt,
-                       n, blocks);
+                       n, packed ? 1u : 0u, blocks);
     return hipGetLastError();
 }
 
@@ -1423,7 +1459,8 @@ static SolveArgs make_args(const SolverLaunch &L) {
              (L.max_sleep & 0xFFu) << 24;
     a.flags = (L.fifo ? 1u : 0u) | (L.hubs && L.split ? 2u : 0u) | (L.stale_check & 3u) << 2 |
               (L.prefetch ? 16u : 0u) | (L.defer_heavy ? 32u : 0u) | (L.light_split ? 64u : 0u) |
-              (L.donate_min & 0xFFu) << 8 | (L.csr_stream ? 1u << 16 : 0u);
+              (L.donate_min & 0xFFu) << 8 | (L.csr_stream ? 1u << 16 : 0u) |
+              (L.packed ? 1u << 17 : 0u);
     a.timeout_ticks = L.timeout_ticks;
     return a;
 }

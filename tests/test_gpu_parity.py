@@ -236,6 +236,63 @@ def test_lane_pairs_and_single_lanes(solver, pop):
             solve_check(solver, row, col, w, src, load=False, l1_pop=pop, **cfg)
 
 
+def back_edge_graph(n, seed):
+    """Degrees 0-12 (inline and CSR vertices side by side): two-way edges with
+    equal or different weights in each direction, parallel edges, one-way
+    edges, self-loops and zero weights."""
+    rng = np.random.default_rng(seed)
+    src, dst, w = [], [], []
+    for u in range(n):
+        for _ in range(int(rng.integers(0, 4 if u % 97 else 9))):
+            v = int(rng.integers(0, n))
+            a, b = int(rng.integers(0, 40)), int(rng.integers(0, 40))
+            kind = int(rng.integers(0, 5))
+            src.append(u), dst.append(v), w.append(a)
+            if kind in (1, 2, 3):   # the reverse edge, same or other weight
+                src.append(v), dst.append(u), w.append(a if kind == 1 else b)
+            if kind == 3:           # a parallel edge
+                src.append(u), dst.append(v), w.append(b)
+        if u % 53 == 0:
+            src.append(u), dst.append(u), w.append(int(rng.integers(0, 5)))
+    src, dst, w = np.array(src), np.array(dst), np.array(w)
+    order = np.lexsort((dst, src))
+    src, dst, w = src[order], dst[order], w[order]
+    row = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(row, src + 1, 1)
+    return np.cumsum(row), dst.astype(np.int32), w.astype(np.int32)
+
+
+@pytest.mark.parametrize("pop", [32, 64])
+def test_back_edges(solver, pop):
+    """A record skips its vertex's edge back to the vertex whose expansion
+    created it (it cannot lower that distance).  Exact on a multigraph with
+    one-way, parallel and zero-weight edges and self-loops, in every phase of
+    the light/heavy split and with the late stale check; on a symmetric grid
+    the skip shows as fewer atomics than examined edges; a weight of 2^28 or
+    more leaves the blocks unpacked and every examined edge is relaxed, in
+    single and batched solves."""
+    say(f"back edges, pop {pop}")
+    row, col, w = back_edge_graph(4000, seed=31)
+    solver.load_csr(row, col, w)
+    for src in (0, 1, 2000):
+        for cfg in (dict(), dict(light_split=1, delta=10, buckets=32, bucket_window=2),
+                    dict(stale_check=3, delta=30), dict(queue="L1V_L2DQ", delta=20),
+                    dict(stale_check=1, prefetch=0)):
+            t, work, _ = solve_check(solver, row, col, w, src, load=False, l1_pop=pop, **cfg)
+            assert t["relaxations"] >= work
+    row, col, w = grid_csr(40, 50, 32)
+    t, work, _ = solve_check(solver, row, col, w, 7, l1_pop=pop)
+    assert t["relaxations"] >= work and t["atomics"] < t["relaxations"]
+    w2 = w.copy()
+    w2[-1] = 1 << 29
+    t, work, _ = solve_check(solver, row, col, w2, 7, l1_pop=pop)
+    assert t["atomics"] == t["relaxations"]
+    sources = [7, 1999, 7, 1000]
+    solver.solve_batch(sources, timeout_s=TIMEOUT, l1_pop=pop, work_blocks=2)
+    for i, src in enumerate(sources):
+        assert np.array_equal(solver.batch_dist(i), O.dijkstra(row, col, w2, src)[0]), f"query {i}"
+
+
 @pytest.mark.parametrize("cfg", [dict(), dict(l1_pop=64), dict(light_split=1, delta=100),
                                  dict(light_split=1, delta=100, l1_pop=64),
                                  dict(queue="L1N_L2DQ"), dict(l1_blocks=1)])
