@@ -63,15 +63,28 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// ---- agent-scope atomics ----
+// ---- atomics ----
+// Loads and stores of queue state take the scope of the waves that share
+// it: agent scope when the workgroups of a launch share it, workgroup scope
+// when one workgroup owns it (the batched kernel's per-query partitions).
+// On gfx950 an agent-scope load refetches its line from memory (sc1), a
+// workgroup-scope load reads through this XCD's L2 (sc0), and an agent-scope
+// acquire or release also invalidates or writes back the XCD's L2; the
+// read-modify-write atomics below are the same instructions at either scope
+// (executed at the memory side), so they keep agent scope.
+constexpr int kAgentScope = __HIP_MEMORY_SCOPE_AGENT;
+constexpr int kGroupScope = __HIP_MEMORY_SCOPE_WORKGROUP;
+template <int S = kAgentScope>
 __device__ __forceinline__ uint32_t ld_relaxed(uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, S);
 }
+template <int S = kAgentScope>
 __device__ __forceinline__ uint32_t ld_acquire(uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, S);
 }
+template <int S = kAgentScope>
 __device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, S);
 }
 __device__ __forceinline__ uint32_t fetch_add(uint32_t *p, uint32_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -104,13 +104,13 @@ __device__ __forceinline__ uint32_t l1_pop_hot(const uint64_t *ring, const HopBl
 
 // Spill every record of the deque to L2 (reference write-back of the whole
 // L1, core/src/l1_queue.cu:332-350).  Returns false if L2 overflowed.
-template <uint32_t kCap = kL1Cap>
+template <uint32_t kCap = kL1Cap, int S = kAgentScope>
 __device__ bool l1_spill_all(const uint64_t *ring, L1State &l1, const L2Queue &q, WaveStats &st) {
     bool ok = true;
     while (l1.size && ok) {
         uint64_t x = kEmpty;
         const uint32_t k = l1_pop<kCap>(ring, l1, kWave, x);
-        ok = l2_push(q, x, lane_id() < k, st);
+        ok = l2_push<S>(q, x, lane_id() < k, st);
     }
     return ok;
 }
@@ -118,7 +118,7 @@ __device__ bool l1_spill_all(const uint64_t *ring, L1State &l1, const L2Queue &q
 // Work donation (no reference counterpart): move up to `want` records from
 // the back of the deque -- the ones SLF would process last -- to L2, where
 // idle waves can claim them.  Returns false if L2 overflowed.
-template <uint32_t kCap = kL1Cap>
+template <uint32_t kCap = kL1Cap, int S = kAgentScope>
 __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, const L2Queue &q,
                           WaveStats &st) {
     l1_uniform(l1);
@@ -130,7 +130,7 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
     wave_sync();
     l1.size -= k;
     st.v[kStatDonated] += k;
-    return l2_push(q, x, lane_id() < k, st);
+    return l2_push<S>(q, x, lane_id() < k, st);
 }
 
 // Push K registers of records: record k of this lane is present when bit k
@@ -148,7 +148,7 @@ __device__ bool l1_donate(const uint64_t *ring, L1State &l1, uint32_t want, cons
 //
 // kBlk selects what goes into the block slots of a block-carrying L1:
 // 0 = no block slots, 1 = the records' blocks `cb[k]`, 2 = kColdMark.
-template <int K, int kBlk = 0, uint32_t kCap = kL1Cap>
+template <int K, int kBlk = 0, uint32_t kCap = kL1Cap, int S = kAgentScope>
 __device__ __forceinline__ bool l1_push(uint64_t *ring, L1State &l1, const uint64_t (&rec)[K],
                                         uint32_t bits, uint32_t spill, uint32_t fifo,
                                         const L2Queue &q, WaveStats &st, HopBlock *blk = nullptr,
@@ -195,7 +195,7 @@ __device__ __forceinline__ bool l1_push(uint64_t *ring, L1State &l1, const uint6
     wave_sync();
     l1.head = (l1.head - total_f) & (kCap - 1);
     l1.size += total;
-    if (l1.size > spill) return l1_spill_all<kCap>(ring, l1, q, st);
+    if (l1.size > spill) return l1_spill_all<kCap, S>(ring, l1, q, st);
     return true;
 }
 

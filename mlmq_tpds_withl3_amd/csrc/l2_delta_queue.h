@@ -30,7 +30,8 @@
 //  * A writer groups its lanes by bucket and reserves every group's slots in
 //    one round trip (the reference's __match_any_sync grouping, done with
 //    ballots, and one fetch_add per bucket leader, all issued together).
-//  * Counters live on separate 128-byte lines; all are agent-scope atomics.
+//  * Counters live on separate 128-byte lines.  Their loads and stores take
+//    the scope S of the waves sharing the queue (device_common.h).
 #pragma once
 
 #include "device_common.h"
@@ -69,20 +70,22 @@ __device__ __forceinline__ uint64_t *slot_ptr(const L2Queue &q, uint32_t b, uint
     return q.slots + ((uint64_t)b << q.cap_log2) + (pos & ((1u << q.cap_log2) - 1));
 }
 
+template <int S = kAgentScope>
 __device__ __forceinline__ void raise_error(const L2Queue &q, uint32_t code) {
     if (lane_id() == 0) {
         cas32(q.ctl + kErrOff, kErrNone, code);
-        st_relaxed(q.ctl + kExitOff, 1u);
+        st_relaxed<S>(q.ctl + kExitOff, 1u);
     }
 }
 
 // Push the records of the lanes with `valid` set.  Wave-uniform call.
 // Returns false when a slot stayed full past the spin bound.
+template <int S = kAgentScope>
 __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &st) {
     uint64_t todo = ballot(valid);
     if (todo == 0) return true;
     const uint32_t lane = lane_id();
-    const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
+    const uint32_t base = uniform(ld_relaxed<S>(q.ctl + kBaseOff));
     uint32_t b = 0;
     if (valid) {
         int32_t rel = (int32_t)(bucket_dist(rec, q.delta) / q.delta) - (int32_t)base;
@@ -123,7 +126,7 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
         }
     }
     if (ballot(!ok)) {
-        raise_error(q, kErrOverflow);
+        raise_error<S>(q, kErrOverflow);
         return false;
     }
     return true;
@@ -140,7 +143,7 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
 // decision needs to know whether any bucket still holds records).
 constexpr uint32_t kNarrowQueue = 8;
 
-template <bool kStrict = false>
+template <bool kStrict = false, int S = kAgentScope>
 __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
                            uint64_t *stage, uint32_t room, WaveStats &st, uint32_t &base_hint) {
     const uint32_t lane = lane_id();
@@ -152,12 +155,12 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
     // that waiting for it does not hold back the others)
     uint32_t t = 0, h = 0, k = 0;
     if (rd) {
-        t = ld_relaxed(q.ctl + kTailOff + lane * kStride);
-        h = ld_relaxed(q.ctl + kHeadOff + lane * kStride);
-        k = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
+        t = ld_relaxed<S>(q.ctl + kTailOff + lane * kStride);
+        h = ld_relaxed<S>(q.ctl + kHeadOff + lane * kStride);
+        k = ld_relaxed<S>(q.ctl + kTakenOff + lane * kStride);
     }
-    const uint32_t base = uniform(ld_relaxed(q.ctl + kBaseOff));
-    const uint32_t done = kStrict && q.strict ? uniform(ld_relaxed(q.ctl + kDoneOff)) : 0u;
+    const uint32_t base = uniform(ld_relaxed<S>(q.ctl + kBaseOff));
+    const uint32_t done = kStrict && q.strict ? uniform(ld_relaxed<S>(q.ctl + kDoneOff)) : 0u;
     const bool unclaimed = rd && (int32_t)(t - h) > 0;
     const uint64_t avail_mask = ballot(unclaimed);
     const uint64_t claim_mask = ballot(mine && cl_mask != 0);
@@ -214,8 +217,8 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
         if (wide && total == 0) {
             uint32_t t2 = 0, k2 = 0;
             if (mine && !rd) {
-                t2 = ld_relaxed(q.ctl + kTailOff + lane * kStride);
-                k2 = ld_relaxed(q.ctl + kTakenOff + lane * kStride);
+                t2 = ld_relaxed<S>(q.ctl + kTailOff + lane * kStride);
+                k2 = ld_relaxed<S>(q.ctl + kTakenOff + lane * kStride);
             }
             total = uniform(wave_sum(t2 - k2));
         }
@@ -235,14 +238,15 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
 // grow and reservations are never below credits, so equality of a later
 // reservation sum with an earlier credit count proves that at the earlier
 // instant no record was queued and no wave held unfinished work.
+template <int S = kAgentScope>
 __device__ bool l2_check_done(const L2Queue &q) {
     const uint32_t lane = lane_id();
-    const uint32_t done = uniform(ld_acquire(q.ctl + kDoneOff));
+    const uint32_t done = uniform(ld_acquire<S>(q.ctl + kDoneOff));
     uint32_t t = 0;
-    if (lane < q.nbuckets) t = ld_relaxed(q.ctl + kTailOff + lane * kStride);
+    if (lane < q.nbuckets) t = ld_relaxed<S>(q.ctl + kTailOff + lane * kStride);
     const uint32_t reserved = uniform(wave_sum(t));
     if (reserved == done) {
-        if (lane == 0) st_relaxed(q.ctl + kExitOff, 1u);
+        if (lane == 0) st_relaxed<S>(q.ctl + kExitOff, 1u);
         return true;
     }
     return false;

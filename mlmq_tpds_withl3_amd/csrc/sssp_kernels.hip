@@ -114,7 +114,7 @@ static_assert(kHeavyUnroll * 64 < 1024, "relax_slots packs per-wave sums in 10-b
 // ids on every lane, active or not) and tentative distances nd[] are loaded;
 // push the improved destinations.  `mid()` runs after the pre-check loads are
 // issued and before they are waited for; any loads it issues stay in flight.
-template <bool kSLF, int kBlk, uint32_t kCap, typename Mid>
+template <bool kSLF, int kBlk, uint32_t kCap, int S, typename Mid>
 __device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act)[kHeavyUnroll],
                                             const uint32_t (&c)[kHeavyUnroll],
                                             const uint32_t (&nd)[kHeavyUnroll], uint64_t *ring,
@@ -163,18 +163,18 @@ __device__ __forceinline__ bool relax_slots(const SolveArgs &a, const bool (&act
     bool ok = true;
     if (sums >> 20) {
         if (kSLF) {
-            ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(),
+            ok = l1_push<kHeavyUnroll, kBlk ? 2 : 0, kCap, S>(ring, l1, out, bits, a.spill(), a.fifo(),
                                                      a.q, st, blk);
         } else {
 #pragma unroll
             for (int j = 0; j < kHeavyUnroll; ++j)
-                if (ok) ok = l2_push(a.q, out[j], (bits >> j) & 1, st);
+                if (ok) ok = l2_push<S>(a.q, out[j], (bits >> j) & 1, st);
         }
     }
     return ok;
 }
 
-template <bool kSLF, int kBlk, uint32_t kCap = kL1Cap>
+template <bool kSLF, int kBlk, uint32_t kCap = kL1Cap, int S = kAgentScope>
 __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t hstart,
                             uint32_t hdeg, uint64_t *ring, HopBlock *blk, L1State &l1,
                             uint32_t *map, WaveStats &st) {
@@ -217,7 +217,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
             const uint32_t nb = pb + kHeavyUnroll;
             bool act2[kHeavyUnroll];
             uint32_t c2[kHeavyUnroll], w2[kHeavyUnroll], o_d2[kHeavyUnroll];
-            ok = relax_slots<kSLF, kBlk, kCap>(a, act, c, nd, ring, blk, l1, st,
+            ok = relax_slots<kSLF, kBlk, kCap, S>(a, act, c, nd, ring, blk, l1, st,
                                          [&]() { load_pieces(nb, act2, c2, w2, o_d2); });
             if (!ok || nb >= ptotal) break;
 #pragma unroll
@@ -270,7 +270,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
         for (int j = 0; j < kHeavyUnroll; ++j) asm volatile("" : "+v"(c[j]), "+v"(nd[j]));
 #pragma unroll
         for (int j = 0; j < kHeavyUnroll; ++j) nd[j] += o_d[j];
-        ok = relax_slots<kSLF, kBlk, kCap>(a, act, c, nd, ring, blk, l1, st, []() {});
+        ok = relax_slots<kSLF, kBlk, kCap, S>(a, act, c, nd, ring, blk, l1, st, []() {});
     }
     return ok;
 }
@@ -279,6 +279,7 @@ __device__ bool relax_heavy(const SolveArgs &a, bool heavy, uint32_t d, uint32_t
 // `nextra` chunks (indices 1..nextra) of hub `hub` at distance `d`.  The
 // chunks are spread over the lanes 64 at a time (owner of a slot = running
 // maximum of start markers, as in relax_heavy) and pushed to L2.
+template <int S = kAgentScope>
 __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32_t nextra,
                             uint32_t *map, WaveStats &st) {
     const uint32_t lane = lane_id();
@@ -299,7 +300,7 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
         const uint32_t od = (uint32_t)__shfl((int)d, o, kWave);
         const uint32_t oe = (uint32_t)__shfl((int)excl, o, kWave);
         const uint64_t rec = make_rec(chunk_vertex_word(oh, 1 + cb + lane - oe), od);
-        if (!l2_push(a.q, rec, act, st)) return false;
+        if (!l2_push<S>(a.q, rec, act, st)) return false;
     }
     return true;
 }
@@ -327,7 +328,7 @@ __device__ bool push_chunks(const SolveArgs &a, uint32_t hub, uint32_t d, uint32
 // results reach the odd lane by a DPP quad permutation, issued with every
 // lane active (a DPP whose source lane is masked off reads 0).
 template <bool kSLF, bool kProf, bool kHot, bool kExt, bool kPre, uint32_t kCap = kL1Cap,
-          bool kPair = false>
+          bool kPair = false, int S = kAgentScope>
 __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bool hot,
                               const uint32_t (&hw)[16], uint64_t *ring, HopBlock *blk, L1State &l1,
                               uint32_t *map, uint32_t (&pf)[kInlineEdges], uint32_t &sunk,
@@ -409,9 +410,9 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     }
     const uint32_t bdeg = kPair ? dpp<0xA0>(b.q[0].x) : b.deg();
     // idle-wave count for work donation, in the same round trip
-    hungry = a.donate_min() ? uniform(ld_relaxed(a.q.ctl + kHungryOff)) : 0u;
+    hungry = a.donate_min() ? uniform(ld_relaxed<S>(a.q.ctl + kHungryOff)) : 0u;
     // bucket epoch, for deferring heavy records (same round trip)
-    const uint32_t epoch = (kExt && a.defer_heavy()) ? uniform(ld_relaxed(a.q.ctl + kBaseOff)) : 0u;
+    const uint32_t epoch = (kExt && a.defer_heavy()) ? uniform(ld_relaxed<S>(a.q.ctl + kBaseOff)) : 0u;
     bool keep = valid && (hot || d <= cur);
     st.v[kStatStale] += popc(ballot((kPair ? lead : valid) && !keep));
     // Deferral: a heavy record whose distance lies beyond the head bucket goes
@@ -422,7 +423,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
                            d / a.q.delta > epoch;
         if (ballot(defer)) {
             st.v[kStatSpare1] += popc(ballot(defer));
-            if (!l2_push(a.q, make_rec(v, d), defer, st)) return false;
+            if (!l2_push<S>(a.q, make_rec(v, d), defer, st)) return false;
             keep = keep && !defer;
         }
     }
@@ -461,7 +462,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     const uint32_t ideg = csr_path ? 0u : deg;
     clk.lap(st, kClkRow);
     if (ballot(nextra > 0)) {
-        if (!push_chunks(a, hub_word - 1, d | (hphase ? kHeavyPhase : 0u), nextra, map, st))
+        if (!push_chunks<S>(a, hub_word - 1, d | (hphase ? kHeavyPhase : 0u), nextra, map, st))
             return false;
     }
 
@@ -593,23 +594,23 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
     bool ok = true;
     if (sums >> 20) {
         if (kSLF) {
-            ok = l1_push<kSlots, kHot ? 1 : 0, kCap>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
+            ok = l1_push<kSlots, kHot ? 1 : 0, kCap, S>(ring, l1, out, bits, a.spill(), a.fifo(), a.q,
                                                st, blk, cb);
         } else {
 #pragma unroll
             for (int k = 0; k < kSlots; ++k)
-                if (ok) ok = l2_push(a.q, out[k], (bits >> k) & 1, st);
+                if (ok) ok = l2_push<S>(a.q, out[k], (bits >> k) & 1, st);
         }
         clk.lap(st, kClkFlush);
     }
     if (ok && ballot(heavy)) {
-        ok = relax_heavy<kSLF, kHot ? 1 : 0, kCap>(a, heavy, d, hstart, deg, ring, blk, l1, map, st);
+        ok = relax_heavy<kSLF, kHot ? 1 : 0, kCap, S>(a, heavy, d, hstart, deg, ring, blk, l1, map, st);
         clk.lap(st, kClkEdges);
     }
     // The heavy edges wait for a heavy-phase record one bucket later: by then
     // the light phase of this bucket has settled the vertex's distance, so
     // the heavy edges are relaxed once instead of once per improvement.
-    if (ok && ballot(need_heavy)) ok = l2_push(a.q, make_rec(vword, d | kHeavyPhase), need_heavy, st);
+    if (ok && ballot(need_heavy)) ok = l2_push<S>(a.q, make_rec(vword, d | kHeavyPhase), need_heavy, st);
     return ok;
 }
 
@@ -619,7 +620,7 @@ __device__ bool process_batch(const SolveArgs &a, uint64_t rec, uint32_t nin, bo
 // termination check while idle.  With lane pairs (kPair) a batch holds at
 // most 32 records, two lanes each: pops and L2 top-ups take at most 32.
 template <bool kSLF, bool kProf, bool kHot = false, bool kExt = false, bool kPre = false,
-          uint32_t kCap = kL1Cap, bool kPair = false>
+          uint32_t kCap = kL1Cap, bool kPair = false, int S = kAgentScope>
 __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint32_t *map,
                            uint64_t *stage, WaveStats &st, PhaseClock<kProf> &clk,
                            uint64_t deadline, HopBlock *blk = nullptr) {
@@ -668,7 +669,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         // a hot batch stays free of global loads.
         if (nin < min(a.min_read(), kRecs) && (!kHot || l1.size == 0)) {
             const uint32_t got =
-                l2_pop<kExt>(q, cl_start, cl_mask, stage, kRecs - nin, st, base_hint);
+                l2_pop<kExt, S>(q, cl_start, cl_mask, stage, kRecs - nin, st, base_hint);
             if (got) {
                 credit += got;
                 const uint32_t r = kPair ? lane >> 1 : lane;
@@ -680,7 +681,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         if (nin == 0) {
             if (credit) {
                 if (lane == 0)
-                    __hip_atomic_fetch_add(q.ctl + kDoneOff, credit, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(q.ctl + kDoneOff, credit, __ATOMIC_RELEASE, S);
                 credit = 0;
             }
             if (a.donate_min() && !idle_listed) {
@@ -688,10 +689,10 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
                 idle_listed = true;
             }
             st.v[kStatIdle] += 1;
-            if (leader && l2_check_done(q)) break;
-            if (uniform(ld_relaxed(q.ctl + kExitOff))) break;
+            if (leader && l2_check_done<S>(q)) break;
+            if (uniform(ld_relaxed<S>(q.ctl + kExitOff))) break;
             if (wall_clock64() > deadline) {
-                raise_error(q, kErrTimeout);
+                raise_error<S>(q, kErrTimeout);
                 break;
             }
             sleep = sleep ? (sleep * 2 < a.max_sleep() ? sleep * 2 : a.max_sleep()) : 1;
@@ -706,18 +707,18 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         }
         st.v[kStatBatches] += 1;
         uint32_t hungry = 0;
-        if (!process_batch<kSLF, kProf, kHot, kExt, kPre, kCap, kPair>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
+        if (!process_batch<kSLF, kProf, kHot, kExt, kPre, kCap, kPair, S>(a, rec, nin, hot, hw, ring, blk, l1, map, pf, sunk, hungry,
                                               st, clk))
             break;
         if (kSLF && hungry && l1.size > a.donate_min()) {
             // (uniform: hungry is a scalar load result)
-            if (!l1_donate<kCap>(ring, l1, uniform(l1.size) / 2, q, st)) break;
+            if (!l1_donate<kCap, S>(ring, l1, uniform(l1.size) / 2, q, st)) break;
             clk.lap(st, kClkFlush);
         }
         if ((st.v[kStatBatches] & 31) == 0) {
-            if (uniform(ld_relaxed(q.ctl + kExitOff))) break;
+            if (uniform(ld_relaxed<S>(q.ctl + kExitOff))) break;
             if (wall_clock64() > deadline) {
-                raise_error(q, kErrTimeout);
+                raise_error<S>(q, kErrTimeout);
                 break;
             }
         }
@@ -830,7 +831,7 @@ __global__ __launch_bounds__(kWaves * 64) void sssp_batch_kernel(BatchArgs b) {
         __threadfence();
         __syncthreads();
         a.dist = b.dist_all + (uint64_t)qi * a.n;
-        run_worker<true, kProf, false, kExt, true, kCap, kPair>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st,
+        run_worker<true, kProf, false, kExt, true, kCap, kPair, kGroupScope>(a, w == 0, s_ring[w], s_map[w], s_stage[w], st,
                                                    clk, deadline);
         __syncthreads();
         if (threadIdx.x == 0) {
