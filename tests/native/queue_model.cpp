@@ -43,7 +43,11 @@ constexpr uint32_t kNarrowQueue = 8;
 
 uint64_t make_rec(uint32_t v, uint32_t d) { return (uint64_t)d << 32 | v; }
 uint32_t rec_v(uint64_t r) { return (uint32_t)r; }
-uint32_t rec_d(uint64_t r) { return (uint32_t)(r >> 32); }
+// Light/heavy split: a heavy-phase record carries this flag in its distance
+// word and is queued one bucket later than its distance.
+constexpr uint32_t kHeavyPhase = 0x80000000u;
+uint32_t rec_d(uint64_t r) { return (uint32_t)(r >> 32) & ~kHeavyPhase; }
+bool rec_heavy(uint64_t r) { return (r >> 32) & kHeavyPhase; }
 
 struct Shared {
     uint32_t nb = 1, window = 1, cap_log2 = 6, delta = 1, batch = 8, spin_limit = 64;
@@ -87,6 +91,7 @@ struct Model {
     int tails_first = 0;    // 1: termination check reads the tails before done
     uint32_t slow_leader = 1;   // wave 0 takes a step 1/slow_leader as often as the others
     uint32_t pop_max = 32, min_read = 8, spill = 64;
+    uint32_t light = 0;   // 1: light/heavy edge split
     // graph (solve mode)
     const int64_t *row = nullptr;
     const int32_t *col = nullptr, *w = nullptr;
@@ -165,7 +170,8 @@ bool l2_push(Wave &wv, const std::vector<uint64_t> &recs) {
     const int n = (int)recs.size();
     std::vector<uint32_t> b(n);
     for (int i = 0; i < n; ++i) {
-        int32_t rel = (int32_t)(rec_d(recs[i]) / q.delta) - (int32_t)base;
+        const uint32_t bd = rec_d(recs[i]) + (rec_heavy(recs[i]) ? q.delta : 0u);
+        int32_t rel = (int32_t)(bd / q.delta) - (int32_t)base;
         rel = rel < 0 ? 0 : rel;
         rel = rel >= (int32_t)q.nb ? (int32_t)q.nb - 1 : rel;
         b[i] = (base + (uint32_t)rel) % q.nb;
@@ -356,12 +362,19 @@ void solve_wave(Wave &wv) {
             continue;
         }
         ++batches;
-        std::vector<uint64_t> out;
+        std::vector<uint64_t> out, heavy;
         for (uint64_t r : in) {
             const uint32_t v = rec_v(r), d = rec_d(r);
+            const bool hphase = m.light && rec_heavy(r);
             const uint32_t cur = ld(&q.dist[v]);
             if (d <= cur) {
+                bool need_heavy = false;
                 for (int64_t e = m.row[v]; e < m.row[v + 1]; ++e) {
+                    if (m.light) {
+                        const bool hv = (uint32_t)m.w[e] >= q.delta;
+                        if (hv && !hphase) need_heavy = true;
+                        if (hv != hphase) continue;
+                    }
                     const uint32_t u = (uint32_t)m.col[e];
                     const uint32_t nd = d + (uint32_t)m.w[e];
                     const uint32_t old = amin(&q.dist[u], nd);
@@ -370,9 +383,19 @@ void solve_wave(Wave &wv) {
                         ++m.outstanding;   // ghost: a new record exists
                     }
                 }
+                if (need_heavy) {
+                    heavy.push_back(make_rec(v, d | kHeavyPhase));
+                    ++m.outstanding;
+                }
             }
         }
         for (uint64_t r : out) l1_push_slf(l1, r);
+        // heavy-phase records go to L2, one bucket later than their distance
+        for (size_t i = 0; i < heavy.size(); i += kLanes) {
+            std::vector<uint64_t> chunk(heavy.begin() + i,
+                                        heavy.begin() + std::min(heavy.size(), i + kLanes));
+            if (!l2_push(wv, chunk)) return;
+        }
         // the batch's records are expanded: their children are held by L1
         m.outstanding -= (int64_t)in.size();
         if (l1.size() > m.spill) {
@@ -438,7 +461,7 @@ bool schedule(Model &m, uint64_t seed) {
         wv.m = &m;
         wv.id = (int)i;
         wv.rng.seed(seed * 7919 + i + 1);
-        wv.stack.assign(1 << 20, 0);
+        wv.stack.assign(1 << 18, 0);
         wv.cl_start.assign(m.q.nb, 0);
         wv.cl_mask.assign(m.q.nb, 0);
         getcontext(&wv.ctx);
@@ -487,17 +510,17 @@ void init_queue(Shared &q, uint32_t nb, uint32_t window, uint32_t cap_log2, uint
 extern "C" {
 
 // One solve of the graph (row int64[n+1], col, w int32) from `src` by
-// `waves` model waves.  cfg: {buckets, window, cap_log2, delta, claim,
+// `waves` (1..512) model waves.  cfg: {buckets, window, cap_log2, delta, claim,
 // spin_limit, pop_max, min_read, spill, advance_rule, tails_first,
 // slow_leader (wave 0 steps 1/slow_leader as often as the others; 0 or 1 = as
-// often)}.
+// often), light (1: light/heavy edge split)}.
 // out (uint64[12]): {finished (0 = step budget ran out), steps, err,
 // exits_raised, exit_violations, sum tail, sum taken, done, slots not empty,
 // outstanding at the end, exit flag, 0}.  dist_out: n distances.
 int qm_solve(int32_t n, const int64_t *row, const int32_t *col, const int32_t *w, int32_t src,
              int32_t waves, const uint32_t *cfg, uint64_t seed, uint64_t max_steps,
              uint32_t *dist_out, uint64_t *out) {
-    if (n <= 0 || src < 0 || src >= n || waves < 1 || waves > 64 || cfg[0] < 1 || cfg[0] > 64 ||
+    if (n <= 0 || src < 0 || src >= n || waves < 1 || waves > 512 || cfg[0] < 1 || cfg[0] > 64 ||
         cfg[1] < 1 || cfg[1] > cfg[0] || cfg[2] > 20 || cfg[3] == 0 || cfg[4] < 1 || cfg[4] > 64)
         return -1;
     Model m;
@@ -509,6 +532,7 @@ int qm_solve(int32_t n, const int64_t *row, const int32_t *col, const int32_t *w
     m.advance_rule = (int)cfg[9];
     m.tails_first = (int)cfg[10];
     m.slow_leader = cfg[11] > 1 ? cfg[11] : 1;
+    m.light = cfg[12];
     m.row = row;
     m.col = col;
     m.w = w;

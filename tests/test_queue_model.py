@@ -54,11 +54,11 @@ SOLVE_OUT = ("finished", "steps", "err", "exits", "exit_violations", "tail", "ta
 
 def model_solve(L, g, src, seed, *, buckets=4, window=2, cap_log2=10, delta=50, claim=8,
                 spin=64, pop=32, min_read=8, spill=64, waves=4, advance_rule=0, tails_first=0,
-                slow_leader=1, max_steps=3_000_000):
+                slow_leader=1, light=0, max_steps=3_000_000):
     row, col, w = (np.ascontiguousarray(x, dtype=t) for x, t in zip(g, (np.int64, np.int32, np.int32)))
     n = len(row) - 1
     cfg = np.array([buckets, window, cap_log2, delta, claim, spin, pop, min_read, spill,
-                    advance_rule, tails_first, slow_leader], dtype=np.uint32)
+                    advance_rule, tails_first, slow_leader, light], dtype=np.uint32)
     dist = np.zeros(n, dtype=np.uint32)
     out = np.zeros(12, dtype=np.uint64)
     rc = L.qm_solve(n, row.ctypes.data, col.ctypes.data, w.ctypes.data, int(src), waves,
@@ -98,6 +98,50 @@ def test_exact_and_quiescent_under_interleavings(qm, gname, shape):
         assert o["tail"] == o["taken"] == o["done"], label
         assert o["full_slots"] == 0 and o["outstanding"] == 0, label
         assert np.array_equal(dist, ref), label
+
+
+LIGHT_SHAPES = [
+    dict(buckets=2, window=1, claim=39, spill=64, pop=25, min_read=10, waves=8, delta=1),
+    dict(buckets=4, window=2, claim=8, spill=4, pop=4, min_read=4, waves=6, delta=30),
+    dict(buckets=64, window=2, claim=64, spill=64, pop=64, min_read=64, waves=5, delta=50),
+]
+
+
+@pytest.mark.parametrize("gname", sorted(GRAPHS))
+@pytest.mark.parametrize("shape", range(len(LIGHT_SHAPES)))
+def test_light_heavy_split_under_interleavings(qm, gname, shape):
+    """The light/heavy split: a record relaxes only its light edges (weight
+    below delta) and, if its vertex has heavy ones, queues a heavy-phase
+    record one bucket later, which relaxes only those.  Exact, quiescent at
+    exit and with every reservation taken and credited, including delta 1
+    with two buckets and a one-bucket window (every edge heavy, every record
+    through L2 twice)."""
+    g = GRAPHS[gname]()
+    ref, _, _ = O.dijkstra(*g, 0)
+    for seed in range(12):
+        dist, o = model_solve(qm, g, 0, seed, light=1, **LIGHT_SHAPES[shape])
+        label = f"{gname} light shape {shape} seed {seed}: {o}"
+        assert o["finished"] == 1 and o["err"] == 0, label
+        assert o["exits"] == 1 and o["exit_violations"] == 0, label
+        assert o["tail"] == o["taken"] == o["done"], label
+        assert o["full_slots"] == 0 and o["outstanding"] == 0, label
+        assert np.array_equal(dist, ref), label
+
+
+def test_many_waves_claims_past_the_ring(qm):
+    """256 waves claiming 39 slots each run far past a 1024-slot ring (claims
+    race ahead of reservations): records still reach exactly one claimant,
+    the solve ends quiescent and exact."""
+    g = hub_csr(400, [300, 120], 800, seed=12, wmax=100)
+    ref, _, _ = O.dijkstra(*g, 5)
+    for seed in range(2):
+        dist, o = model_solve(qm, g, 5, seed, buckets=2, window=1, cap_log2=10, delta=1, claim=39,
+                              pop=25, min_read=10, spill=64, waves=256, light=1, max_steps=20_000_000)
+        label = f"seed {seed}: {o}"
+        assert o["finished"] == 1 and o["exits"] == 1 and o["exit_violations"] == 0, label
+        if o["err"] == 0:
+            assert o["tail"] == o["taken"] == o["done"] and o["outstanding"] == 0, label
+            assert np.array_equal(dist, ref), label
 
 
 EARLY = dict(buckets=4, window=1, claim=1, spill=4, pop=4, min_read=1, waves=2, delta=5,
