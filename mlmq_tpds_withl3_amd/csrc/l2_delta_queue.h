@@ -134,7 +134,12 @@ __device__ bool l2_push(const L2Queue &q, uint64_t rec, bool valid, WaveStats &s
 
 // Pop up to one claim's worth (at most `room`) of ready records into
 // `stage[0..got)`.  `cl_start`/`cl_mask` carry the wave's claims: lane b
-// holds bucket b.
+// holds bucket b.  `recs` is the most records one batch of the wave holds (the
+// room of an idle wave); a new claim takes at most that many slots, so an idle
+// wave tries every slot it still holds.  A wider claim can strand records: its
+// lowest slots may be emptied for good by claims one ring lap ahead (made on a
+// stale snapshot, they take the records under them), and the records under
+// its higher slots, which no other claim covers, are then never tried.
 // Queues with more buckets than this read, per pop, only the counters of
 // the window and one bucket past it, placed by the epoch the wave saw last
 // (`base_hint`): every wave polls these lines, so reading 3 x 64 of them per
@@ -145,7 +150,8 @@ constexpr uint32_t kNarrowQueue = 8;
 
 template <bool kStrict = false, int S = kAgentScope>
 __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_mask,
-                           uint64_t *stage, uint32_t room, WaveStats &st, uint32_t &base_hint) {
+                           uint64_t *stage, uint32_t room, uint32_t recs, WaveStats &st,
+                           uint32_t &base_hint) {
     const uint32_t lane = lane_id();
     const bool mine = lane < q.nbuckets;
     const bool wide = q.nbuckets > kNarrowQueue && !(kStrict && q.strict);
@@ -173,7 +179,7 @@ __device__ uint32_t l2_pop(const L2Queue &q, uint32_t &cl_start, uint64_t &cl_ma
         uint64_t mk = read_lane64(cl_mask, b);
         if (!has_claim) {
             const uint32_t avail = read_lane(t, b) - read_lane(h, b);
-            const uint32_t kk = avail < q.batch ? avail : q.batch;
+            const uint32_t kk = min(avail, min(q.batch, recs));
             uint32_t s0 = 0;
             if (lane == 0) s0 = fetch_add(q.ctl + kHeadOff + b * kStride, kk);
             s = read_lane(s0, 0);

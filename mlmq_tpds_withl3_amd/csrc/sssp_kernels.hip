@@ -669,7 +669,7 @@ __device__ void run_worker(const SolveArgs &a, bool leader, uint64_t *ring, uint
         // a hot batch stays free of global loads.
         if (nin < min(a.min_read(), kRecs) && (!kHot || l1.size == 0)) {
             const uint32_t got =
-                l2_pop<kExt, S>(q, cl_start, cl_mask, stage, kRecs - nin, st, base_hint);
+                l2_pop<kExt, S>(q, cl_start, cl_mask, stage, kRecs - nin, kRecs, st, base_hint);
             if (got) {
                 credit += got;
                 const uint32_t r = kPair ? lane >> 1 : lane;

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(64) void publication_kernel(L2Queue q, uint32_t pro
     uint64_t cl_mask = 0;
     for (uint32_t it = 0; it < max_iters; ++it) {
         if (uniform(ld_relaxed(counts + 1)) >= total) break;
-        const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, kWave, st, base_hint);
+        const uint32_t got = l2_pop(q, cl_start, cl_mask, stage, kWave, kWave, st, base_hint);
         if (lane < got) {
             const uint64_t r = stage[lane];
             const uint32_t v = rec_vertex(r);

@@ -17,12 +17,14 @@
 // cannot observe: the number of records created and not yet fully expanded
 // must be zero at the instant the termination check raises exit.
 //
-// Variants reproduce the two protocol choices this build made differently
-// from its first version and from the reference:
+// Variants reproduce protocol choices that the device code makes differently
+// from its earlier versions and from the reference:
 //   * advance_rule 1: a wave holding a claim on the head bucket does not
 //     advance the window (the rule before the fix: it can stall);
 //   * tails_first 1: the termination check reads the reservations before the
-//     credits (the reference's order: it can exit early).
+//     credits (the reference's order: it can exit early);
+//   * wide_claims 1: a claim may cover more slots than the wave's batch holds
+//     (the rule before the fix: the ring can lap under it and strand records).
 #include <ucontext.h>
 
 #include <algorithm>
@@ -82,6 +84,9 @@ struct Model {
     int cur = -1;
     uint64_t steps = 0, max_steps = 0;
     bool out_of_steps = false;
+    // scripted interleaving: the first steps go to these waves in order (an
+    // entry naming a finished wave is drawn at random instead)
+    std::vector<int32_t> script;
     // ghost state
     int64_t outstanding = 0;        // records created and not yet fully expanded
     uint64_t exit_violations = 0;   // exits raised while work was outstanding
@@ -92,6 +97,10 @@ struct Model {
     uint32_t slow_leader = 1;   // wave 0 takes a step 1/slow_leader as often as the others
     uint32_t pop_max = 32, min_read = 8, spill = 64;
     uint32_t light = 0;   // 1: light/heavy edge split
+    uint32_t room = kLanes;   // publication mode: records a consumer takes per pop
+    // 1: a claim takes `batch` slots even where that is more than the wave's
+    // batch holds (the earlier rule, which can strand records; see l2_pop)
+    uint32_t wide_claims = 0;
     // graph (solve mode)
     const int64_t *row = nullptr;
     const int32_t *col = nullptr, *w = nullptr;
@@ -217,8 +226,10 @@ bool l2_push(Wave &wv, const std::vector<uint64_t> &recs) {
     return true;
 }
 
-// l2_pop: up to `room` records into `stage`.
-uint32_t l2_pop(Wave &wv, std::vector<uint64_t> &stage, uint32_t room) {
+// l2_pop: up to `room` records into `stage`.  `recs` is the most records a
+// batch of this wave holds (the room of an idle wave): a new claim takes at
+// most that many slots, so an idle wave tries every slot it still holds.
+uint32_t l2_pop(Wave &wv, std::vector<uint64_t> &stage, uint32_t room, uint32_t recs) {
     Shared &q = g_model->q;
     const uint32_t nb = q.nb;
     const bool wide = nb > kNarrowQueue;
@@ -256,7 +267,8 @@ uint32_t l2_pop(Wave &wv, std::vector<uint64_t> &stage, uint32_t room) {
         uint64_t mk = wv.cl_mask[b];
         if (!had_claim[b]) {
             const uint32_t avail = t[b] - h[b];
-            const uint32_t kk = avail < q.batch ? avail : q.batch;
+            uint32_t kk = avail < q.batch ? avail : q.batch;
+            if (!g_model->wide_claims) kk = std::min(kk, recs);
             s = fadd(&q.head[b], kk);
             mk = kk >= 64 ? ~0ull : ((1ull << kk) - 1);
         }
@@ -342,13 +354,16 @@ void solve_wave(Wave &wv) {
     for (;;) {
         if (m.stop()) return;
         std::vector<uint64_t> in;
-        while (!l1.empty() && in.size() < m.pop_max) {
+        // A batch holds at most 64 records, or 32 when a pop takes at most 32
+        // (the device then runs two lanes per record): L2 top-ups stop there too.
+        const uint32_t recs = m.pop_max <= 32 ? 32u : (uint32_t)kLanes;
+        while (!l1.empty() && in.size() < std::min(m.pop_max, recs)) {
             in.push_back(l1.front());
             l1.pop_front();
         }
-        if (in.size() < m.min_read) {
+        if (in.size() < std::min(m.min_read, recs)) {
             std::vector<uint64_t> stage;
-            const uint32_t got = l2_pop(wv, stage, kLanes - (uint32_t)in.size());
+            const uint32_t got = l2_pop(wv, stage, recs - (uint32_t)in.size(), recs);
             credit += got;
             in.insert(in.end(), stage.begin(), stage.end());
         }
@@ -431,7 +446,7 @@ void publication_wave(Wave &wv) {
         if (m.stop()) return;
         if (ld(&m.consumed) >= m.total) return;
         std::vector<uint64_t> stage;
-        const uint32_t got = l2_pop(wv, stage, kLanes);
+        const uint32_t got = l2_pop(wv, stage, m.room, m.room);
         for (uint64_t r : stage) {
             const uint32_t v = rec_v(r);
             if (v < m.total && rec_d(r) == dist_of(v, m.dist0, m.span)) ++m.seen[v];
@@ -479,6 +494,10 @@ bool schedule(Model &m, uint64_t seed) {
         if (live.empty()) break;
         if (++m.steps > m.max_steps) m.out_of_steps = true;   // waves now return at their next check
         int pick = live[rng() % live.size()];
+        if (m.steps <= m.script.size()) {
+            const int32_t sp = m.script[m.steps - 1];
+            if (sp >= 0 && (size_t)sp < m.waves.size() && !m.waves[(size_t)sp].finished) pick = sp;
+        }
         // a slow leader: its turn is passed on to another wave (drawn again)
         // with probability 1 - 1/slow_leader
         if (pick == 0 && live.size() > 1 && m.slow_leader > 1 && rng() % m.slow_leader != 0)
@@ -513,7 +532,8 @@ extern "C" {
 // `waves` (1..512) model waves.  cfg: {buckets, window, cap_log2, delta, claim,
 // spin_limit, pop_max, min_read, spill, advance_rule, tails_first,
 // slow_leader (wave 0 steps 1/slow_leader as often as the others; 0 or 1 = as
-// often), light (1: light/heavy edge split)}.
+// often), light (1: light/heavy edge split), wide_claims (1: the earlier claim
+// rule)}.
 // out (uint64[12]): {finished (0 = step budget ran out), steps, err,
 // exits_raised, exit_violations, sum tail, sum taken, done, slots not empty,
 // outstanding at the end, exit flag, 0}.  dist_out: n distances.
@@ -533,6 +553,7 @@ int qm_solve(int32_t n, const int64_t *row, const int32_t *col, const int32_t *w
     m.tails_first = (int)cfg[10];
     m.slow_leader = cfg[11] > 1 ? cfg[11] : 1;
     m.light = cfg[12];
+    m.wide_claims = cfg[13];
     m.row = row;
     m.col = col;
     m.w = w;
@@ -560,20 +581,26 @@ int qm_solve(int32_t n, const int64_t *row, const int32_t *col, const int32_t *w
 
 // The device queue check's scenario: `producers` waves push `batch` records
 // each while `consumers` waves pop until all are consumed.  cfg as qm_solve
-// (pop/min_read/spill/tails_first unused), plus cfg[11] = initial epoch.
+// (pop/min_read/spill/tails_first unused), plus cfg[11] = initial epoch,
+// cfg[12] = records a consumer takes per pop (1..64; 0 = 64), cfg[13] as
+// qm_solve's wide_claims.
 // out (uint64[8]): {finished, steps, consumed, bad, push_fail, records seen
 // != 1, sum tail, sum taken}.
-int qm_publication(uint32_t producers, uint32_t batch, uint32_t consumers, uint32_t dist0,
-                   uint32_t span, const uint32_t *cfg, uint64_t seed, uint64_t max_steps,
-                   uint64_t *out) {
+// `script` (may be null): the wave (producers first, then consumers) that
+// takes each of the first `script_len` steps.
+int qm_publication_scripted(uint32_t producers, uint32_t batch, uint32_t consumers, uint32_t dist0,
+                            uint32_t span, const uint32_t *cfg, uint64_t seed, uint64_t max_steps,
+                            const int32_t *script, uint32_t script_len, uint64_t *out) {
     if (producers < 1 || consumers < 1 || producers + consumers > 256 || batch < 1 || batch > 64 ||
         span == 0 || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > cfg[0] || cfg[2] > 20 ||
-        cfg[3] == 0 || cfg[4] < 1 || cfg[4] > 64)
+        cfg[3] == 0 || cfg[4] < 1 || cfg[4] > 64 || cfg[12] > 64)
         return -1;
     Model m;
     g_mode = 1;
     init_queue(m.q, cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[11]);
     m.advance_rule = (int)cfg[9];
+    m.room = cfg[12] ? cfg[12] : (uint32_t)kLanes;
+    m.wide_claims = cfg[13];
     m.producers = producers;
     m.pub_batch = batch;
     m.dist0 = dist0;
@@ -581,6 +608,7 @@ int qm_publication(uint32_t producers, uint32_t batch, uint32_t consumers, uint3
     m.total = producers * batch;
     m.seen.assign(m.total, 0);
     m.max_steps = max_steps;
+    if (script) m.script.assign(script, script + script_len);
     m.waves.resize(producers + consumers);
     const bool fin = schedule(m, seed);
     uint64_t not_once = 0, t = 0, k = 0;
@@ -592,6 +620,13 @@ int qm_publication(uint32_t producers, uint32_t batch, uint32_t consumers, uint3
     const uint64_t o[8] = {fin ? 1u : 0u, m.steps, m.consumed, m.bad, m.push_fail, not_once, t, k};
     std::memcpy(out, o, sizeof o);
     return 0;
+}
+
+int qm_publication(uint32_t producers, uint32_t batch, uint32_t consumers, uint32_t dist0,
+                   uint32_t span, const uint32_t *cfg, uint64_t seed, uint64_t max_steps,
+                   uint64_t *out) {
+    return qm_publication_scripted(producers, batch, consumers, dist0, span, cfg, seed, max_steps,
+                                   nullptr, 0, out);
 }
 
 }  // extern "C"

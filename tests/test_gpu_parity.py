@@ -236,6 +236,33 @@ def test_lane_pairs_and_single_lanes(solver, pop):
             solve_check(solver, row, col, w, src, load=False, l1_pop=pop, **cfg)
 
 
+@pytest.mark.parametrize("case", ["rmat", "hub"])
+def test_lane_pair_claims_wider_than_a_batch(solver, case):
+    """Lane pairs (a batch holds 32 records) with L2 claims of 63 and 39 slots,
+    512 work blocks and delta 1 on small graphs, so the bucket rings lap many
+    times: a new claim is cut to the 32 slots a batch holds, so an idle wave
+    tries every slot it holds and no record is left under an untried slot
+    (a wider claim can strand records and spin to the watchdog).  Exact on
+    repeated solves from several sources."""
+    say(f"lane-pair claims wider than a batch, {case}")
+    if case == "rmat":
+        g = HostGraph.rmat(12, 14, seed=940935275)
+        row, col, w = (x.copy() for x in g.arrays())
+        g.close()
+        srcs = (3328, 0, 17)
+        cfg = dict(delta=1, buckets=4, bucket_window=3, l2_batch=63, min_read=31, l1_pop=16,
+                   stale_check=2, l1_spill=16, idle_backoff=1)
+    else:
+        row, col, w = hub_csr(5000, [4000, 1500], 10000, seed=616, wmax=1000)
+        srcs = (1702, 1, 4999)
+        cfg = dict(delta=1, buckets=2, bucket_window=1, l2_batch=39, min_read=10, l1_pop=25,
+                   stale_check=3, light_split=1, l1_spill=64, idle_backoff=16)
+    solver.load_csr(row, col, w)
+    for rep in range(3):
+        for src in srcs:
+            solve_check(solver, row, col, w, src, load=False, work_blocks=512, prefetch=1, **cfg)
+
+
 def back_edge_graph(n, seed):
     """Degrees 0-12 (inline and CSR vertices side by side): two-way edges with
     equal or different weights in each direction, parallel edges, one-way

@@ -19,6 +19,9 @@ scheduler, so each seed is one reproducible interleaving.  Checked here:
   cu_delta_queue.cuh:591-600) lets the check exit early, and the window
   rule from before the fix (a claim on the head bucket holds the window
   back) stalls the device queue check's scenario;
+* claims no larger than a batch: a claim of more slots than a wave's batch
+  holds lets the ring lap under it and strand records in a scripted
+  interleaving; capped at the batch, the same interleaving drains;
 * overflow: a ring too small for the work ends every wave with the overflow
   error instead of losing records.
 """
@@ -45,6 +48,8 @@ def qm():
     L.qm_solve.restype = ctypes.c_int
     L.qm_publication.argtypes = [u32] * 5 + [P, u64, u64, P]
     L.qm_publication.restype = ctypes.c_int
+    L.qm_publication_scripted.argtypes = [u32] * 5 + [P, u64, u64, P, u32, P]
+    L.qm_publication_scripted.restype = ctypes.c_int
     return L
 
 
@@ -54,11 +59,11 @@ SOLVE_OUT = ("finished", "steps", "err", "exits", "exit_violations", "tail", "ta
 
 def model_solve(L, g, src, seed, *, buckets=4, window=2, cap_log2=10, delta=50, claim=8,
                 spin=64, pop=32, min_read=8, spill=64, waves=4, advance_rule=0, tails_first=0,
-                slow_leader=1, light=0, max_steps=3_000_000):
+                slow_leader=1, light=0, wide_claims=0, max_steps=3_000_000):
     row, col, w = (np.ascontiguousarray(x, dtype=t) for x, t in zip(g, (np.int64, np.int32, np.int32)))
     n = len(row) - 1
     cfg = np.array([buckets, window, cap_log2, delta, claim, spin, pop, min_read, spill,
-                    advance_rule, tails_first, slow_leader, light], dtype=np.uint32)
+                    advance_rule, tails_first, slow_leader, light, wide_claims], dtype=np.uint32)
     dist = np.zeros(n, dtype=np.uint32)
     out = np.zeros(12, dtype=np.uint64)
     rc = L.qm_solve(n, row.ctypes.data, col.ctypes.data, w.ctypes.data, int(src), waves,
@@ -172,9 +177,9 @@ PUB_OUT = ("finished", "steps", "consumed", "bad", "push_fail", "not_once", "tai
 
 def model_publication(L, seed, advance_rule, *, producers=32, batch=64, consumers=4,
                       buckets=16, window=4, cap_log2=12, delta=100, claim=64, span=3200, base=0,
-                      max_steps=400_000):
-    cfg = np.array([buckets, window, cap_log2, delta, claim, 1 << 16, 0, 0, 0, advance_rule, 0, base],
-                   dtype=np.uint32)
+                      room=0, wide_claims=0, spin=1 << 16, max_steps=400_000):
+    cfg = np.array([buckets, window, cap_log2, delta, claim, spin, 0, 0, 0, advance_rule, 0, base,
+                    room, wide_claims], dtype=np.uint32)
     out = np.zeros(8, dtype=np.uint64)
     rc = L.qm_publication(producers, batch, consumers, 0, span, cfg.ctypes.data, seed, max_steps,
                           out.ctypes.data)
@@ -199,6 +204,57 @@ def test_window_rule_before_the_fix_stalls(qm):
         o = model_publication(qm, seed, advance_rule=0)
         assert o["finished"] == 1, (seed, o)
         assert o["consumed"] == o["tail"] == o["taken"] == 2048, (seed, o)
+        assert o["bad"] == o["push_fail"] == o["not_once"] == 0, (seed, o)
+
+
+def lapped_claim_script():
+    """Step-by-step interleaving of 4 producers x 16 records (waves 0-3) and
+    consumers X, Y, Y2, Z (waves 4-7) taking 32 records per pop, on one bucket
+    of 64 slots with claims of 48:
+    producers 0-2 reserve positions 0..47; X claims all 48 and polls its lowest
+    32 slots before any is written (a wave's first step only starts it; then a
+    push is 1 epoch load, 1 tail fetch_add and one CAS per record, and a pop 1
+    load of the consumed counter, 4 counter loads, 1 head fetch_add, one
+    exchange per tried slot, and 2 fetch_adds when it took records); the
+    producers write; producer 3 reserves and writes 48..63; Y, Y2 and Z read
+    the counters (head 48, tail 64); Z claims 48..63 and takes them; Y and Y2,
+    on their stale snapshot, claim 64..79 and 80..95, one lap ahead, whose
+    slots hold records 0..31, and take them.  X's lowest 32 slots then stay
+    empty for good, and records 32..47 sit in slots only X's claim covers."""
+    sc = [0] * 3 + [1] * 3 + [2] * 3
+    sc += [4] * (1 + 1 + 4 + 1 + 32)
+    sc += [0] * 16 + [1] * 16 + [2] * 16
+    sc += [3] * (3 + 16)
+    sc += [5] * 6 + [6] * 6 + [7] * 6
+    sc += [7] * 19 + [5] * 19 + [6] * 19
+    return np.array(sc, dtype=np.int32)
+
+
+def test_claims_larger_than_a_batch_strand_records(qm):
+    """A wave tries at most the slots its batch can hold (32 records with lane
+    pairs), lowest first.  With claims of more slots than that, the scripted
+    interleaving (lapped_claim_script) strands 16 records under a claim whose
+    tried slots the ring has emptied: no wave takes them again, and the solve
+    would spin until the watchdog.  With claims capped at the batch size (the
+    device rule) the same interleaving, and random ones, drain every record."""
+    sc = lapped_claim_script()
+    cfg = lambda wide: np.array([1, 1, 6, 1, 48, 1 << 16, 0, 0, 0, 0, 0, 0, 32, wide], dtype=np.uint32)
+    out = np.zeros(8, dtype=np.uint64)
+    for wide, seed in ((1, 0), (1, 1), (0, 0), (0, 1)):
+        c = cfg(wide)
+        assert qm.qm_publication_scripted(4, 16, 4, 0, 1, c.ctypes.data, seed, 200_000, sc.ctypes.data,
+                                          len(sc), out.ctypes.data) == 0
+        o = dict(zip(PUB_OUT, (int(x) for x in out)))
+        if wide:
+            assert o["finished"] == 0 and o["consumed"] == 48 and o["taken"] == 48, o
+            assert o["tail"] == 64 and o["push_fail"] == 0, o
+        else:
+            assert o["finished"] == 1 and o["consumed"] == o["taken"] == o["tail"] == 64, o
+            assert o["bad"] == o["push_fail"] == o["not_once"] == 0, o
+    for seed in range(20):
+        o = model_publication(qm, seed, 0, producers=4, batch=16, consumers=4, buckets=1, window=1,
+                              cap_log2=6, delta=1, claim=48, span=1, room=32, max_steps=400_000)
+        assert o["finished"] == 1 and o["consumed"] == o["tail"] == 64, (seed, o)
         assert o["bad"] == o["push_fail"] == o["not_once"] == 0, (seed, o)
 
 
