@@ -74,7 +74,8 @@ typedef struct mlmq_params {
     int32_t  blocks_per_cu;   /* used when work_blocks == 0 */
     int32_t  buckets;         /* delta buckets (reference BNUM), 1..64, default 16 */
     int32_t  bucket_window;   /* buckets readable at once (BUCKET_MAX), default 4 */
-    int32_t  l2_batch;        /* records per L2 claim (l2_batch_size), 1..64, default 8 */
+    int32_t  l2_batch;        /* records per L2 claim (l2_batch_size), 1..64, default 8;
+                                 at most 32 when l1_pop <= 32 (a batch holds 32 records) */
     int32_t  min_read;        /* top up from L2 below this many (MIN_READ_GRA), 1..64, default 8
                                  (0 is refused: no wave would ever take the seed) */
     int64_t  bucket_capacity; /* ring slots per bucket (rounded up to 2^k); 0 = auto */
