@@ -4,10 +4,12 @@ batched-solve parameters drawn within its documented range (queue modes,
 buckets and window, claims of 1..64 with and without lane pairs, top-up and
 pop sizes, stale-check modes, child-block touches, light/heavy and hub
 splits, L1 spill, grid sizes up to 512 workgroups), each result compared
-bit-exactly with the oracle.  A solve may end with a queue overflow (reported,
-then skipped) but never with a watchdog expiry or a wrong distance: sweeps of
-this kind found the lapped-claim stall of lane-pair waves (claims wider than
-a batch), which this test keeps covered."""
+bit-exactly with the oracle; and served requests over random resident graphs
+with random per-graph parameters and server settings, in both serving modes.
+A solve may end with a queue overflow (reported, then skipped) but never with
+a watchdog expiry or a wrong distance: sweeps of this kind found the
+lapped-claim stall of lane-pair waves (claims wider than a batch), which this
+test keeps covered."""
 import sys
 
 import numpy as np
@@ -79,6 +81,56 @@ def draw_batch_params(rng):
              work_blocks=int(rng.choice([0, 1, 2])), timeout_s=TIMEOUT)
     p["bucket_window"] = int(rng.integers(1, p["buckets"] + 1))
     return p
+
+
+def draw_serve_params(rng):
+    b = int(rng.choice([1, 2, 4, 16, 64]))
+    return dict(delta=int(rng.choice([5, 50, 300, 5000, 200000])), buckets=b,
+                bucket_window=int(rng.integers(1, b + 1)), l2_batch=int(rng.integers(1, 65)),
+                min_read=int(rng.integers(1, 65)), l1_pop=int(rng.integers(1, 65)),
+                l1_spill=int(rng.integers(1, 65)), workgroups=int(rng.integers(1, 9)),
+                waves=int(rng.choice([1, 4, 8, 16])), hub_split=int(rng.integers(0, 2)),
+                light_split=int(rng.integers(0, 2)), prefetch=int(rng.choice([-1, 1])),
+                stale_check=int(rng.choice([-1, 1, 2, 3])))
+
+
+@pytest.mark.parametrize("seed", [51, 52])
+def test_randomized_serving_sweep(seed):
+    """Served requests over random resident graphs, per-graph parameters drawn
+    within their documented ranges, launch lanes or the live launch, random
+    launch sizes and spreading: every request completes and is exact."""
+    from mlmq_tpds_withl3_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    for rnd in range(16):
+        mode = str(rng.choice(["lanes", "live"]))
+        cfg = dict(hbm_budget=512 << 20, streams=int(rng.integers(1, 5)), max_batch=int(rng.integers(1, 33)),
+                   spread_waves=int(rng.choice([0, 64, 512])), timeout_s=TIMEOUT, mode=mode)
+        graphs, params = [], []
+        s = Server(0, **cfg)
+        try:
+            for _ in range(3):
+                kind, gseed, g = draw_graph(rng)
+                p = draw_serve_params(rng)
+                gid = s.add_graph(g, **p)
+                graphs.append((kind, gseed, g))
+                params.append(p)
+                assert gid == len(graphs) - 1
+            nreq = 24
+            gsel = rng.integers(0, 3, size=nreq).astype(np.int32)
+            src = np.array([int(rng.integers(0, len(graphs[gi][2][0]) - 1)) for gi in gsel], dtype=np.int32)
+            arrival = np.sort(rng.uniform(0.0, 0.005, size=nreq))
+            outs = [np.empty(len(graphs[gi][2][0]) - 1, dtype=np.int32) for gi in gsel]
+            res, _ = s.run(arrival, gsel, src, dist_out=outs)
+        finally:
+            s.close()
+        for i, (gi, sv) in enumerate(zip(gsel, src)):
+            kind, gseed, (row, col, w) = graphs[gi]
+            case = f"round {rnd} {cfg} request {i}: graph kind {kind} seed {gseed} src {sv} {params[gi]}"
+            assert res["status"][i] == 0, f"{case}: status {res['status'][i]}"
+            ref = O.dijkstra(row, col, w, int(sv))[0]
+            bad = np.flatnonzero(outs[i] != ref)
+            assert bad.size == 0, f"{case}: {bad.size} mismatches"
+        say(f"serving seed {seed} round {rnd} ({mode}): {nreq} requests exact")
 
 
 @pytest.mark.parametrize("seed", [41, 42])
