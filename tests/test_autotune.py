@@ -102,9 +102,9 @@ def test_tune_batch_descends_every_coordinate():
 def test_tune_batch_moves_pop_and_top_up_together():
     # best: pop 32 with top-up 32 (claim 32); raising either alone costs more
     def cost(d):
-        c = 3.0 + (0.0 if d["l1_pop"] == d["min_read"] == 32 else 0.5)
-        c += 1.0 if d["min_read"] > d["l1_pop"] else 0.0
-        return c + (0.1 if d["l1_pop"] > d["min_read"] else 0.0)
+        c = 3.0 + (0.0 if d["l1_pop"] == d["min_read"] == 32 else 1.0)
+        c += 2.0 if d["min_read"] > d["l1_pop"] else 0.0
+        return c + (0.5 if d["l1_pop"] > d["min_read"] else 0.0)
 
     s = FakeSolver(cost)
     r = A.tune_batch(s, np.arange(4), base=dict(batch_waves=16, l1_pop=16, min_read=16, l2_batch=16))
