@@ -146,8 +146,8 @@ WORKLOADS = {
     # 264,346 points, 3-NN, weight = max(1, round(len * 1e5)), source 0, delta 200000
     "road264k": dict(kind="knn", n=264_346, k=3, seed=1, weight_scale=1e5, morton=False,
                      source="zero", delta=200_000, delta_sweep=(200_000,),
-                     bench_point=dict(delta=200_000, buckets=4, bucket_window=4, l2_batch=8,
-                                      min_read=8, l1_pop=32, work_blocks=256, stale_check=3)),
+                     bench_point=dict(delta=200_000, buckets=2, bucket_window=2, l2_batch=4,
+                                      min_read=4, l1_pop=32, work_blocks=192, stale_check=3)),
     # 6,262,104 points, 3-NN, Morton order, longest edge -> 100,000, centroid source
     "road6m": dict(kind="knn", n=6_262_104, k=3, seed=2, weight_scale=-100_000.0, morton=True,
                    source="centroid", delta=200_000,

@@ -193,7 +193,7 @@ def test_road264k(solver):
     row, col, w = g.arrays()
     for cfg in (dict(), dict(delta=20000), dict(queue="L1N_L2DQ"), dict(queue="L1V_L2DQ"),
                 dict(stale_check=1), dict(stale_check=0), dict(stale_check=3), dict(l1_blocks=1),
-                dict(light_split=1, delta=20000)):
+                dict(light_split=1, delta=20000), dict(spec["bench_point"])):
         say(f"road264k {cfg}")
         t, work, reached = solve_check(solver, row, col, w, src, **cfg)
         assert t["relaxations"] >= work
