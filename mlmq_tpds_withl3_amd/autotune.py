@@ -31,7 +31,7 @@ def tune_single(solver, src, deltas, grid=None, repeats: int = 2, confirm: int =
     """Grid search for single queries from `src` (one source or a list of
     sources: a point's time is then the mean over them): every delta in
     `deltas` crossed with `grid` (a list of parameter dicts, default: L1 pop
-    size / L2 top-up threshold pairs, bucket counts and three grid sizes),
+    size / L2 top-up threshold pairs, bucket counts and four grid sizes),
     then the late and L2-cached stale checks at the four best points and
     child-block touches off at the four best after that.  Each
     point is the best device time of `repeats` solves per source; the four
@@ -45,8 +45,9 @@ def tune_single(solver, src, deltas, grid=None, repeats: int = 2, confirm: int =
     if grid is None:
         grid = [dict(l1_pop=pop, min_read=mr, l2_batch=lb, work_blocks=wb, buckets=nb,
                      bucket_window=min(4, nb))
-                for wb in (256, 384, 512) for nb in (16, 4, 2)
-                for pop, mr, lb in ((32, 32, 8), (32, 32, 32), (32, 16, 16), (32, 8, 8), (64, 8, 8))]
+                for wb in (192, 256, 384, 512) for nb in (16, 4, 2)
+                for pop, mr, lb in ((32, 32, 8), (32, 32, 32), (32, 16, 16), (32, 8, 8), (32, 4, 4),
+                                    (64, 8, 8))]
     table = []
 
     def measure(params):

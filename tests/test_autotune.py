@@ -110,7 +110,7 @@ def test_tune_batch_moves_pop_and_top_up_together():
     r = A.tune_batch(s, np.arange(4), base=dict(batch_waves=16, l1_pop=16, min_read=16, l2_batch=16))
     p = r["params"]
     assert p["l1_pop"] == p["min_read"] == p["l2_batch"] == 32, p
-    assert p["stale_check"] == 1
+    assert r["trail"][0][0]["stale_check"] == 1   # the search starts at the cached stale check
 
 
 def test_tune_batch_revisits_delta_after_the_shape_moves():
