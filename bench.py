@@ -591,12 +591,14 @@ def run_serve(args):
         log(f"serve graph {gid} {name}: n={g.n} m={g.m} pool={len(pool)} "
             f"slot={server.slot_bytes(gid) / 2**20:.1f} MiB ({time.time() - t0:.1f}s)")
         # one-at-a-time comparator: plain solves of the pool sources, tuned
-        # like the single-query mode; its parameters also seed the serving tuner
+        # like the single-query mode; its parameters (or the workload's serve_point)
+        # seed the serving tuner
         with Solver(0) as sv:
             sv.load_csr(row, col, w)
             from mlmq_tpds_withl3_amd.autotune import tune_single
             single = dict(spec.get("bench_point") or
                           tune_single(sv, pool[:3], spec["delta_sweep"], grid=spec.get("tune_grid"))["params"])
+            seed = dict(spec.get("serve_point") or single)
             sp = default_params(timeout_s=60.0, **single)
             sv.solve(pool[0], sp)
             t1 = time.perf_counter()
@@ -605,7 +607,7 @@ def run_serve(args):
             seq_ms.append((time.perf_counter() - t1) * 1e3 / len(pool))
         serve_keys = ("delta", "buckets", "bucket_window", "l2_batch", "min_read", "l1_pop",
                       "light_split")
-        base = {k: v for k, v in single.items() if k in serve_keys}
+        base = {k: v for k, v in seed.items() if k in serve_keys}
         if args.serve_parts:
             parts = int(args.serve_parts.split(",")[gi])
             base.update(workgroups=max(1, parts // 8), waves=8 if parts >= 8 else 1)

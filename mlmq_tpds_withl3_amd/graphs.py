@@ -147,7 +147,12 @@ WORKLOADS = {
     "road264k": dict(kind="knn", n=264_346, k=3, seed=1, weight_scale=1e5, morton=False,
                      source="zero", delta=200_000, delta_sweep=(200_000,),
                      bench_point=dict(delta=200_000, buckets=2, bucket_window=2, l2_batch=4,
-                                      min_read=4, l1_pop=32, work_blocks=192, stale_check=3)),
+                                      min_read=4, l1_pop=32, work_blocks=192, stale_check=3),
+                     # seed of the serving tuner: a launch holds many requests, where the
+                     # single query's claims of 4 cost latency (p50 at 1000 queries/s
+                     # 14.6 -> 19.7 ms untuned)
+                     serve_point=dict(delta=200_000, buckets=4, bucket_window=4, l2_batch=8,
+                                      min_read=8, l1_pop=32)),
     # 6,262,104 points, 3-NN, Morton order, longest edge -> 100,000, centroid source
     "road6m": dict(kind="knn", n=6_262_104, k=3, seed=2, weight_scale=-100_000.0, morton=True,
                    source="centroid", delta=200_000,
